@@ -1,0 +1,215 @@
+/*
+ * orthoroute_hip.h -- C ABI of the MI355X (gfx950) PathFinder / SSSP library
+ * (liborx_hip.so).
+ *
+ * Plain C: every buffer is a raw device pointer the caller owns (the Python
+ * host keeps them in PyTorch-ROCm tensors); the library never frees caller
+ * memory and never throws across the ABI.  Every entry point takes the HIP
+ * stream to enqueue on and returns an int status:
+ *     ORX_OK (0), ORX_NO_PATH (1), or a negative ORX_E_* error;
+ * orx_last_error() returns the message of the last failure on this thread.
+ *
+ * Each declaration names the interface of bbenchoff/OrthoRoute it replaces
+ * (file:line; UP = orthoroute/algorithms/manhattan/unified_pathfinder.py,
+ * CD = orthoroute/algorithms/manhattan/pathfinder/cuda_dijkstra.py,
+ * PK = orthoroute/algorithms/manhattan/pathfinder/persistent_kernel.py).
+ */
+#ifndef ORTHOROUTE_HIP_H
+#define ORTHOROUTE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* orx_stream_t; /* a hipStream_t; NULL = default stream */
+
+enum {
+    ORX_OK = 0,
+    ORX_NO_PATH = 1,
+    ORX_E_ARG = -1,        /* bad argument / shape                          */
+    ORX_E_HIP = -2,        /* HIP runtime error (launch, attribute)         */
+    ORX_E_OVERFLOW = -3,   /* a queue, path or hop counter exceeded its cap */
+    ORX_E_CYCLE = -4,      /* parent walk did not reach a root              */
+    ORX_E_NODEVICE = -5
+};
+
+/* ---------------------------------------------------------------------- */
+/* Lattice state resident in HBM.  N = X*Y*L nodes, node id z*X*Y+y*X+x   */
+/* (UP:118, kicad_geometry.py:59-62).  Four arc slots per node, ordered by */
+/* neighbour id: 0 via down, 1 planar minus, 2 planar plus, 3 via up; arc  */
+/* id 4*u+s; the reverse of slot s is 3-s.  Bit z of hmask set = layer z   */
+/* routes along x (H), else along y (V) (UP:1487-1495).  Absent arcs have  */
+/* base = +inf.  Replaces CSRGraph + EdgeAccountant arrays (UP:851-1002,   */
+/* UP:1039-1052) and the via / path-node pools (UP:3019-3048, 4623-4659).   */
+/* ---------------------------------------------------------------------- */
+typedef struct orx_lattice {
+    int32_t X, Y, L;
+    uint32_t hmask;
+    const float* base;   /* [4N] base arc cost                               */
+    float* hist;         /* [4N] arc history                                 */
+    int32_t* occ;        /* [4N] arc occupancy (both arcs of an edge equal)  */
+    float* wcost;        /* [4N] live entering weight fl(c_arc + q_head)     */
+    float* node_hist;    /* [N]                                              */
+    int32_t* node_use;   /* [N]  path-node use                               */
+    int32_t* via_col;    /* [X*Y] via barrels per column                     */
+} orx_lattice;
+
+/* Cost / history parameters (UP:1301-1384 update_costs, 1226-1289
+ * update_history, 5308-5383 node penalty).  The arc cost is
+ *   fl(fl(fl(b*base_w) + fl(hist_w*h)) + fl(pres_fac*max(0, occ+1-edge_cap)))
+ * the node term fl(fl(node_hist_w*nh) + fl(node_pres*max(0, use+1-node_cap))),
+ * and the entering weight their fp32 sum. */
+typedef struct orx_params {
+    float pres_fac;
+    float base_w;
+    float hist_w;
+    float node_pres;
+    float node_hist_w;
+    int32_t edge_cap;
+    int32_t node_cap;
+    float hist_gain;
+    float hist_decay;
+    float hist_cap_mult;   /* > 0: increment <= mult*base (UP:1255-1262)   */
+    float node_hist_gain;
+} orx_params;
+
+/* Per-net tables of one routing job (resident in HBM). */
+typedef struct orx_nets {
+    int32_t n;
+    const int32_t* box;       /* [4n] ROI box x0, y0, w, h (all layers)      */
+    const int32_t* src;       /* [n] global source node                     */
+    const int32_t* dst;       /* [n] global sink node                       */
+    int32_t* path_arena;      /* concatenated node paths, src first         */
+    const int64_t* path_off;  /* [n] offset of each net's path              */
+    const int32_t* path_cap;  /* [n] capacity (>= w*h*L)                    */
+    int32_t* path_len;        /* [n] 0 = unrouted                           */
+    float* path_cost;         /* [n] fp32 distance of the sink              */
+    int32_t* status;          /* [n] last status (ORX_OK / ORX_NO_PATH / <0) */
+} orx_nets;
+
+/* Scratch for one wave: every slot k of the wave owns nodes
+ * [ws_off[k], ws_off[k] + R_k) of keys (u64), 4 queues (u32, stride
+ * ws_nodes) and 3 bitmaps (u32 words, offset ws_off[k]/32).  ws_off must be
+ * multiples of 64.  Sizes: keys 8*ws_nodes B, queues 16*ws_nodes B,
+ * bitmaps 3*ws_nodes/8 B. */
+typedef struct orx_workspace {
+    uint64_t* keys;
+    uint32_t* queues;
+    uint32_t* bitmaps;
+    int64_t ws_nodes;
+} orx_workspace;
+
+typedef struct orx_route_cfg {
+    float delta;            /* bucket width; +inf = pure frontier relaxation */
+    int32_t block;          /* workgroup size: 256, 512 or 1024              */
+    int32_t lds_nodes;      /* ROIs up to this many nodes keep keys in LDS   */
+    int32_t flags;          /* ORX_F_* */
+    int64_t* stats;         /* device [8] accumulators or NULL               */
+} orx_route_cfg;
+
+#define ORX_F_STATS 1       /* count settled nodes / relaxed arcs per query  */
+#define ORX_F_FULL 2        /* no early exit: settle the whole ROI           */
+#define ORX_F_SEED_PEN 4    /* CSR seeds pay their own node penalty (multi-  */
+                            /* seed forms UP:2582-2583, CD:5589-5606)        */
+
+/* stats[]: 0 settled nodes (label-setting count up to the sink), 1 relaxed
+ * arcs (their in-ROI out-degree), 2 queue items processed, 3 near rounds,
+ * 4 threshold advances, 5 nets routed, 6 nets failed, 7 path hops. */
+
+/* Library init: checks for a gfx950 device.  Replaces the CuPy
+ * RawKernel JIT set-up of CD:98-150 / PK:519-529. */
+int orx_init(void);
+const char* orx_last_error(void);
+int orx_version(void);
+
+/* Recompute wcost for every arc from base/hist/occ/node arrays (the
+ * per-iteration cost refresh: UP:1301-1384 update_costs followed by
+ * UP:1386-1401 begin_live_present_costs). */
+int orx_lattice_reprice(const orx_lattice* lat, const orx_params* p, orx_stream_t stream);
+
+/* End-of-iteration accounting, fused: overuse over canonical arcs
+ * (UP:1137-1190, 4919-4932), node overuse (UP:10718-10726), via hops and
+ * barrels, then the history update (UP:1226-1289 with raw present, node
+ * history UP:10048-10081).  d_out (device int64[8]) receives edge_over,
+ * edge_over_count, node_over, node_over_count, via_hops, barrels; it is
+ * zeroed by the call. */
+int orx_lattice_end_iteration(const orx_lattice* lat, const orx_params* p, int64_t* d_out,
+                              orx_stream_t stream);
+
+/* Route one wave of nets whose ROI boxes are pairwise disjoint, one
+ * workgroup per net: rip up the old path, delta-stepping SSSP over the ROI
+ * box, parent backtrace, commit, live re-price of touched arcs.  Equivalent
+ * to running the per-net body of _route_all (UP:7949-8430) for these nets in
+ * any order.  d_wave_nets[K] are net ids, d_ws_off[K] workspace offsets;
+ * max_nodes = max ROI size in the wave. */
+int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* nets,
+                   const int32_t* d_wave_nets, const int64_t* d_ws_off, int32_t K,
+                   int64_t max_nodes, const orx_workspace* ws, const orx_route_cfg* cfg,
+                   orx_stream_t stream);
+
+/* Per-net overuse of its committed path, summed over the forward arcs
+ * (d_over[n]) and its hop count (d_hops[n]): the congestion input of
+ * _order_nets_by_difficulty (UP:7840-7901). */
+int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_cap,
+                    int64_t* d_over, int32_t* d_hops, orx_stream_t stream);
+
+/* Commit (sign=+1) or rip up (sign=-1) K device paths into the occupancy
+ * arrays and re-price the touched arcs (EdgeAccountant.commit_path /
+ * clear_path + refresh_live_present_costs, UP:1074-1135, 1403-1422;
+ * _mark/_clear_path_node_use UP:4635-4659; via columns UP:4470-4546).
+ * Paths must have pairwise distinct arcs across the batch. */
+int orx_apply_paths(const orx_lattice* lat, const orx_params* p, const int32_t* d_paths,
+                    const int64_t* d_off, const int32_t* d_len, int32_t K, int32_t sign,
+                    orx_stream_t stream);
+
+/* One box-ROI query on the current wcost (the SSSP microbenchmark and the
+ * array-reporting form of find_path_roi_gpu, CD:4951-5040).  Writes
+ * ROI-local dist (fp32) and parent (global id, -1 root / unreached) arrays.
+ * Small ROIs run in one workgroup; larger ones run grid-wide. */
+int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int32_t h,
+                 int64_t src, int64_t dst, float* d_dist, int32_t* d_parent,
+                 const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream);
+
+/* Generic CSR query behind the gpu_solver seam:
+ *   find_path_roi_gpu(src, dst, costs, roi_nodes, global_to_roi, node_penalty)
+ *     (CD:4951-5040; UP:2465-2480), and
+ *   find_path_fullgraph_gpu_seeds(costs, src_seeds, dst_targets, ...)
+ *     (CD:5516-5850, PK:260-516).
+ * The graph must carry in_rank[e] (rank of the arc's source among its
+ * target's in-neighbours ordered by (source, arc)) and the transposed
+ * in_indptr / in_src.  roi_nodes / g2r NULL = whole graph; pen indexed by
+ * ROI index (NULL = none); allowed bitmap over global ids (NULL = all).
+ * Enqueues the search; d_key[R] receives the packed dist/hops/slot keys and
+ * d_best[0] the chosen target (global id, -1 none).  d_scratch needs
+ * 4*R + 4*ceil(R/32) words. */
+typedef struct orx_csr {
+    int64_t N, E;
+    const int32_t* indptr;
+    const int32_t* indices;
+    const int32_t* in_rank;
+    const int32_t* in_indptr;
+    const int32_t* in_src;
+} orx_csr;
+
+int orx_sssp_csr(const orx_csr* g, const float* d_costs, int64_t R, const int32_t* d_roi_nodes,
+                 const int32_t* d_g2r, const float* d_pen, const uint32_t* d_allowed,
+                 int32_t n_seeds, const int32_t* d_seeds, const float* d_seed_cost,
+                 int32_t n_targets, const int32_t* d_targets, const float* d_target_cost,
+                 uint64_t* d_key, uint32_t* d_scratch, int64_t scratch_words, int32_t* d_best,
+                 const orx_route_cfg* cfg, orx_stream_t stream);
+
+/* Parent walk of a CSR query result (backtrace_parent CD:5857-5900,
+ * backtrace_paths CD:1845-1950) from the target in d_target[0] (device, e.g.
+ * the d_best of orx_sssp_csr): writes the global path, root first, into
+ * d_path[cap]; d_len[0] = length, ORX_NO_PATH or a negative status. */
+int orx_backtrace_csr(const orx_csr* g, const uint64_t* d_key, const int32_t* d_g2r,
+                      const int32_t* d_target, int32_t* d_path, int32_t cap, int32_t* d_len,
+                      orx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORTHOROUTE_HIP_H */

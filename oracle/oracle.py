@@ -1,0 +1,291 @@
+"""Python side of the CPU parity oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker; the product path (orthoroute_amd) never
+imports it.  It wraps liborx_oracle.so (oracle/orx_oracle.c, a C restatement
+of the reference CPU path; see that file's header for the reference lines it
+follows) and adds pure-Python twins for tiny cases.
+
+Parity status of each oracle function (how it is pinned, see tests/):
+  sssp_box / pathfinder SSSP  -- pinned: brute-force Bellman-Ford fixpoint on
+      tiny lattices, scipy.sparse.csgraph.dijkstra (float64, exact for dyadic
+      costs), closed-form uncongested distances, fixpoint invariants, and the
+      pure-Python heapq twin below;
+  sssp_csr -- pinned: the reference's own vectors (tests/test_gpu_fallback.py:
+      seed pricing [1, 3], fp32 absorption chain [3, 2, 1, 0], random 6x6 grid
+      seed 20260727 vs float64 heapq, ROI-CSR folded weights [7.0, 3.0]);
+  accountant (commit / clear / overuse / history / cost) -- pinned:
+      tests/unit/test_edge_accountant.py, tests/test_via_accounting.py,
+      tests/test_lattice.py:255-276 vectors, restated;
+  net order (_order_nets_by_difficulty) -- host code shared with the product
+      (orthoroute_amd.waves), pinned by construction tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import heapq
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liborx_oracle.so"
+
+KEY_INF = 0xFFFFFFFFFFFFFFFF
+ROOT_SLOT = 0xFF
+
+
+class OracleParams(C.Structure):
+    _fields_ = [
+        ("pres_fac", C.c_float), ("base_w", C.c_float), ("hist_w", C.c_float),
+        ("node_pres", C.c_float), ("node_hist_w", C.c_float),
+        ("edge_cap", C.c_int32), ("node_cap", C.c_int32),
+        ("hist_gain", C.c_float), ("hist_decay", C.c_float),
+        ("hist_cap_mult", C.c_float), ("node_hist_gain", C.c_float),
+    ]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    P = C.c_void_p
+    lib.orx_oracle_pathfinder.restype = C.c_int
+    lib.orx_oracle_pathfinder.argtypes = [
+        C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, P, P, P, C.c_int32, P, P,
+        C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int64]
+    lib.orx_oracle_sssp_box.restype = C.c_int
+    lib.orx_oracle_sssp_box.argtypes = [
+        C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+        C.c_int64, C.c_int64, C.c_int32, P, P, P, P]
+    lib.orx_oracle_sssp_csr.restype = C.c_int
+    lib.orx_oracle_sssp_csr.argtypes = [
+        C.c_int64, P, P, P, P, P, P, C.c_int64, P, P, P, P, C.c_int32, C.c_int32, P, P, C.c_int32, P, P, P, P, P,
+        P]
+    lib.orx_oracle_overuse.restype = None
+    lib.orx_oracle_overuse.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_int32, C.c_int32, P]
+    lib.orx_oracle_update_history.restype = None
+    lib.orx_oracle_update_history.argtypes = [C.c_int64, P, P, P, C.c_int64, P, P, C.POINTER(OracleParams)]
+    lib.orx_oracle_lattice_weights.restype = None
+    lib.orx_oracle_lattice_weights.argtypes = [
+        C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, P, P, P, P, C.POINTER(OracleParams), P]
+    lib.orx_oracle_net_overuse.restype = None
+    lib.orx_oracle_net_overuse.argtypes = [
+        C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, C.c_int32, P, P, P, P, P]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def params_from(cfg, pres: float) -> OracleParams:
+    p = np.float32(pres)
+    return OracleParams(float(p), cfg.base_w, cfg.hist_w, float(np.float32(cfg.node_pres_w) * p),
+                        cfg.node_hist_w, cfg.edge_cap, cfg.node_cap, cfg.hist_gain, cfg.hist_decay,
+                        cfg.hist_cap_mult, cfg.node_hist_gain)
+
+
+class OraclePathFinder:
+    """Sequential CPU PathFinder over box ROIs with the same host order logic
+    as the device driver (orthoroute_amd.router.DevicePathFinder)."""
+
+    def __init__(self, netlist, cfg, base: np.ndarray | None = None):
+        from orthoroute_amd.waves import net_distance
+        self.lib = load()
+        self.netlist = netlist
+        self.cfg = cfg
+        spec = netlist.spec
+        self.spec = spec
+        N = spec.N
+        self.base = (spec.base_costs() if base is None else np.asarray(base, np.float32)).copy()
+        self.hist = np.zeros(4 * N, np.float32)
+        self.occ = np.zeros(4 * N, np.int32)
+        self.node_hist = np.zeros(N, np.float32)
+        self.node_use = np.zeros(N, np.int32)
+        self.via_col = np.zeros(spec.XY, np.int32)
+        R = netlist.roi_nodes().astype(np.int64)
+        self.path_cap = R
+        self.path_off = np.zeros(netlist.n, np.int64)
+        if netlist.n:
+            self.path_off[1:] = np.cumsum(R)[:-1]
+        self.arena = np.zeros(max(1, int(R.sum())), np.int32)
+        self.path_len = np.zeros(netlist.n, np.int32)
+        self.path_cost = np.full(netlist.n, np.inf, np.float32)
+        self.box = np.ascontiguousarray(netlist.box.reshape(-1), np.int32)
+        self.src = netlist.src.astype(np.int32)
+        self.dst = netlist.dst.astype(np.int32)
+        self.distance = net_distance(spec, netlist.src, netlist.dst)
+        self.schedule = cfg.schedule(spec.L)
+        self.iteration = 0
+        self.history = []
+
+    def net_overuse(self):
+        n = self.netlist.n
+        over = np.zeros(n, np.int64)
+        hops = np.zeros(n, np.int32)
+        s = self.spec
+        self.lib.orx_oracle_net_overuse(s.X, s.Y, s.L, s.hmask, _p(self.occ), self.cfg.edge_cap, n,
+                                        _p(self.arena), _p(self.path_off), _p(self.path_len), _p(over), _p(hops))
+        return over, hops
+
+    def order_for(self, iteration: int) -> np.ndarray:
+        from orthoroute_amd.waves import net_order
+        if iteration > 1:
+            over, hops = self.net_overuse()
+        else:
+            over = np.zeros(self.netlist.n, np.int64)
+            hops = np.zeros(self.netlist.n, np.int32)
+        return net_order(iteration, self.distance, over, hops)
+
+    def run_iteration(self, order: np.ndarray | None = None, nets_limit: int = -1) -> dict:
+        it = self.iteration + 1
+        if order is None:
+            order = self.order_for(it)
+        s = self.spec
+        pres = np.asarray([self.schedule[it - 1]], np.float32)
+        order = np.ascontiguousarray(order, np.int32)
+        stats = np.zeros(16, np.int64)
+        P = params_from(self.cfg, 1.0)
+        P.node_pres = float(np.float32(self.cfg.node_pres_w))  # scaled by pres inside
+        rc = self.lib.orx_oracle_pathfinder(
+            s.X, s.Y, s.L, s.hmask, _p(self.base), self.netlist.n, _p(self.src), _p(self.dst), _p(self.box),
+            1, _p(order), _p(pres), C.byref(P), _p(self.hist), _p(self.occ), _p(self.node_hist),
+            _p(self.node_use), _p(self.via_col), _p(self.arena), _p(self.path_off), _p(self.path_cap),
+            _p(self.path_len), _p(self.path_cost), _p(stats), int(nets_limit))
+        if rc < 0:
+            raise RuntimeError(f"oracle pathfinder failed: {rc}")
+        self.iteration = it
+        keys = ["routed", "failed", "edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops",
+                "barrels", "settled", "relaxed", "path_hops"]
+        d = {k: int(stats[i]) for i, k in enumerate(keys)}
+        d["iteration"] = it
+        d["pres_fac"] = float(pres[0])
+        self.history.append(d)
+        return d
+
+    def paths(self) -> list[np.ndarray]:
+        return [self.arena[o:o + l].copy() for o, l in zip(self.path_off, self.path_len)]
+
+    def state(self) -> dict:
+        return {"hist": self.hist, "occ": self.occ, "node_hist": self.node_hist, "node_use": self.node_use,
+                "via_col": self.via_col}
+
+
+def lattice_weights(spec, base, hist, occ, node_hist, node_use, params: OracleParams) -> np.ndarray:
+    lib = load()
+    out = np.empty(4 * spec.N, np.float32)
+    lib.orx_oracle_lattice_weights(spec.X, spec.Y, spec.L, spec.hmask, _p(base), _p(hist), _p(occ),
+                                   _p(node_hist), _p(node_use), C.byref(params), _p(out))
+    return out
+
+
+def sssp_box(spec, wcost: np.ndarray, box, src: int, dst: int = -1, full: bool = True):
+    """One box query on precomputed weights: (dist, parent, hops, settled, relaxed)."""
+    lib = load()
+    x0, y0, w, h = (int(v) for v in box)
+    R = w * h * spec.L
+    dist = np.empty(R, np.float32)
+    parent = np.empty(R, np.int32)
+    hops = np.empty(R, np.int32)
+    stats = np.zeros(2, np.int64)
+    wc = np.ascontiguousarray(wcost, np.float32)
+    rc = lib.orx_oracle_sssp_box(spec.X, spec.Y, spec.L, spec.hmask, _p(wc), x0, y0, w, h, int(src), int(dst),
+                                 1 if full else 0, _p(dist), _p(parent), _p(hops), _p(stats))
+    if rc < 0:
+        raise RuntimeError(f"oracle sssp_box failed: {rc}")
+    return dist, parent, hops, int(stats[0]), int(stats[1])
+
+
+def sssp_csr(N, indptr, indices, costs, in_rank, in_indptr, in_src, roi_nodes=None, g2r=None, pen=None,
+             allowed=None, seeds=(), seed_cost=None, targets=(), tg_cost=None, seed_pen=True):
+    """Generic CSR query: returns (rc, best_target, dist[R], parent_roi[R]).
+    seed_pen: seeds pay their own node penalty (multi-seed forms); the
+    single-source find_path_roi form passes False."""
+    lib = load()
+    a = lambda x, t: None if x is None else np.ascontiguousarray(x, t)  # noqa: E731
+    indptr, indices = a(indptr, np.int32), a(indices, np.int32)
+    costs, in_rank = a(costs, np.float32), a(in_rank, np.int32)
+    in_indptr, in_src = a(in_indptr, np.int32), a(in_src, np.int32)
+    roi_nodes, g2r, pen = a(roi_nodes, np.int32), a(g2r, np.int32), a(pen, np.float32)
+    allowed = a(allowed, np.uint32)
+    seeds, targets = a(seeds, np.int32), a(targets, np.int32)
+    seed_cost, tg_cost = a(seed_cost, np.float32), a(tg_cost, np.float32)
+    R = N if roi_nodes is None else len(roi_nodes)
+    dist = np.empty(R, np.float32)
+    parent = np.empty(R, np.int32)
+    best = np.full(1, -1, np.int32)
+    stats = np.zeros(2, np.int64)
+    rc = lib.orx_oracle_sssp_csr(N, _p(indptr), _p(indices), _p(costs), _p(in_rank), _p(in_indptr), _p(in_src),
+                                 R, _p(roi_nodes), _p(g2r), _p(pen), _p(allowed), 1 if seed_pen else 0,
+                                 len(seeds), _p(seeds),
+                                 _p(seed_cost), len(targets), _p(targets), _p(tg_cost), _p(dist), _p(parent),
+                                 _p(best), _p(stats))
+    if rc < 0:
+        raise RuntimeError(f"oracle sssp_csr failed: {rc}")
+    return rc, int(best[0]), dist, parent
+
+
+def csr_path(best: int, parent_roi: np.ndarray, roi_nodes=None, g2r=None) -> list[int]:
+    """Walk ROI-local parents from the chosen target (global ids, root first)."""
+    if best < 0:
+        return []
+    r = int(g2r[best]) if g2r is not None else best
+    out = []
+    while r >= 0:
+        out.append(int(roi_nodes[r]) if roi_nodes is not None else r)
+        r = int(parent_roi[r])
+    return out[::-1]
+
+
+# ---------------------------------------------------------------------------
+# Pure-Python twin (tiny cases only): heapq Dijkstra with the canonical key
+# ---------------------------------------------------------------------------
+def py_box_dijkstra(spec, wcost: np.ndarray, box, src: int):
+    """Pure-Python canonical Dijkstra over a box (full sweep).  Returns
+    {global node: (dist, hops, parent_global or -1)}."""
+    x0, y0, w, h = (int(v) for v in box)
+    nb = spec.neighbours()
+
+    def inside(g):
+        x, y, _ = spec.coords(g)
+        return x0 <= x < x0 + w and y0 <= y < y0 + h
+
+    best = {src: (np.float32(0.0), 0, ROOT_SLOT)}
+    heap = [(np.float32(0.0), 0, ROOT_SLOT, src)]
+    done = set()
+    while heap:
+        d, hp, sl, u = heapq.heappop(heap)
+        if u in done or best[u] != (d, hp, sl):
+            continue
+        done.add(u)
+        for s in range(4):
+            v = int(nb[u, s])
+            if v < 0 or not inside(v) or v in done:
+                continue
+            wt = np.float32(wcost[4 * u + s])
+            if np.isinf(wt):
+                continue
+            nd = np.float32(d + wt)
+            key = (nd, hp + 1, 3 - s)
+            if v not in best or key < best[v]:
+                best[v] = key
+                heapq.heappush(heap, (nd, hp + 1, 3 - s, v))
+    out = {}
+    for v, (d, hp, sl) in best.items():
+        par = -1 if sl == ROOT_SLOT else int(nb[v, sl])
+        out[v] = (float(d), hp, par)
+    return out

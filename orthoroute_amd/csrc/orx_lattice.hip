@@ -1,0 +1,802 @@
+// orx_lattice.hip -- lattice PathFinder kernels for gfx950.
+//
+// route_wave_kernel: one workgroup per net of a disjoint-ROI wave.  Each
+// workgroup runs the whole per-net body of the reference's _route_all
+// (unified_pathfinder.py:7949-8430) on its ROI box:
+//   rip-up of the old path (EdgeAccountant.clear_path UP:1118-1135 and
+//   _clear_path_node_use UP:4648-4659), live re-price of the touched arcs
+//   (refresh_live_present_costs UP:1403-1422), delta-stepping SSSP over the
+//   ROI (replaces the wavefront search CD:2800-3533 and the heap Dijkstra
+//   UP:2451-2548), parent backtrace (CD:1845-1950), commit (UP:1091-1116,
+//   4635-4646, via columns UP:4470-4546) and re-price again.
+// Nets of one wave have pairwise disjoint ROI boxes, so their commits touch
+// disjoint arcs and the wave is equivalent to routing them sequentially.
+//
+// SSSP: near-far delta-stepping.  Keys live in LDS when the ROI fits
+// (lds_nodes), else in the global workspace.  The near frontier is
+// deduplicated with a per-round bitmap and appended with wave ballots + a
+// prefix popcount (one LDS atomic per wave); the far pile is deduplicated
+// with a third bitmap and re-split when the near bucket [.., T) drains:
+// T <- min(far) + delta.  With early exit the search stops once the sink's
+// distance is below T (all keys below T are final).
+#include "orx_common.hpp"
+#include "../../include/orthoroute_hip.h"
+
+using namespace orx;
+
+namespace {
+
+struct NetsDev {
+    int32_t n;
+    const int32_t* box;
+    const int32_t* src;
+    const int32_t* dst;
+    int32_t* path_arena;
+    const int64_t* path_off;
+    const int32_t* path_cap;
+    int32_t* path_len;
+    float* path_cost;
+    int32_t* status;
+};
+
+struct Ctl {
+    uint32_t qcnt[3];
+    uint32_t fcnt;
+    uint32_t fcnt2;
+    uint32_t minbits;
+    int32_t err;
+    int32_t path_len;
+    uint32_t rounds, phases, items;
+    unsigned long long settled, relaxed;
+};
+
+struct Box {
+    int32_t x0, y0, w, h, L;
+    uint32_t wh, R;
+    FastDiv dw, dwh;
+};
+
+// Atomic helpers (workgroup scope: every access to an ROI's search state
+// comes from the one workgroup that owns the ROI).
+__device__ __forceinline__ uint64_t key_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint64_t key_min(uint64_t* p, uint64_t v) {
+    return __hip_atomic_fetch_min((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t bit_set(uint32_t* bm, uint32_t i) {
+    uint32_t b = 1u << (i & 31u);
+    return __hip_atomic_fetch_or(bm + (i >> 5), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & b;
+}
+__device__ __forceinline__ void bit_clear(uint32_t* bm, uint32_t i) {
+    __hip_atomic_fetch_and(bm + (i >> 5), ~(1u << (i & 31u)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Decode an ROI-local index.
+__device__ __forceinline__ void roi_decode(const Box& B, uint32_t r, uint32_t& lx, uint32_t& ly, uint32_t& z) {
+    z = B.dwh.div(r);
+    uint32_t rem = r - z * B.wh;
+    ly = B.dw.div(rem);
+    lx = rem - ly * (uint32_t)B.w;
+}
+
+// ROI-local neighbour of r in slot s (UINT32_MAX when outside the box).
+__device__ __forceinline__ uint32_t roi_nbr(const Box& B, bool horiz, uint32_t r, uint32_t lx, uint32_t ly,
+                                            uint32_t z, int s) {
+    switch (s) {
+    case 0: return z > 0 ? r - B.wh : UINT32_MAX;
+    case 1: return horiz ? (lx > 0 ? r - 1 : UINT32_MAX) : (ly > 0 ? r - (uint32_t)B.w : UINT32_MAX);
+    case 2: return horiz ? (lx + 1 < (uint32_t)B.w ? r + 1 : UINT32_MAX)
+                         : (ly + 1 < (uint32_t)B.h ? r + (uint32_t)B.w : UINT32_MAX);
+    default: return z + 1 < (uint32_t)B.L ? r + B.wh : UINT32_MAX;
+    }
+}
+
+__device__ __forceinline__ int64_t roi_global(const Lat& g, const Box& B, uint32_t lx, uint32_t ly, uint32_t z) {
+    return (int64_t)z * g.XY + (int64_t)(B.y0 + (int32_t)ly) * g.X + (B.x0 + (int32_t)lx);
+}
+
+// ---------------------------------------------------------------------------
+// Accountant pieces run by one workgroup over one path.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__device__ void apply_path_block(const Lat& g, const int32_t* path, int32_t len, int32_t sign) {
+    for (int32_t i = threadIdx.x; i < len; i += BLOCK) atomicAdd(g.node_use + path[i], sign);
+    for (int32_t i = threadIdx.x; i + 1 < len; i += BLOCK) {
+        int64_t u = path[i], v = path[i + 1];
+        int s = lat_slot(g, u, v);
+        atomicAdd(g.occ + 4 * u + s, sign);
+        atomicAdd(g.occ + 4 * v + (3 - s), sign);
+        bool vert = (s == 0 || s == 3);
+        bool prev_vert = false;
+        if (i > 0) {
+            int ps = lat_slot(g, path[i - 1], u);
+            prev_vert = (ps == 0 || ps == 3);
+        }
+        if (vert && !prev_vert) atomicAdd(g.via_col + (u % g.XY), sign);
+    }
+}
+
+// Re-price every in-arc of every path node: covers both arcs of every hop
+// (each hop arc's head is a path node) and the node-term change.
+__device__ __forceinline__ void reprice_in_arcs(const Lat& g, const Params& P, int64_t v) {
+    float q = node_cost(g.node_hist[v], __hip_atomic_load(g.node_use + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P);
+    for (int t = 0; t < 4; ++t) {
+        int64_t n = lat_nbr(g, v, t);
+        if (n < 0) continue;
+        int64_t a = 4 * n + (3 - t);
+        float b = g.base[a];
+        float w = INFINITY;
+        if (!isinf(b)) {
+            int32_t o = __hip_atomic_load(g.occ + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w = __fadd_rn(arc_cost(b, g.hist[a], o, P), q);
+        }
+        g.wcost[a] = w;
+    }
+}
+
+template <int BLOCK>
+__device__ void reprice_path_block(const Lat& g, const Params& P, const int32_t* path, int32_t len) {
+    for (int32_t i = threadIdx.x; i < len; i += BLOCK) reprice_in_arcs(g, P, path[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Near-far delta-stepping over one ROI box, one workgroup.
+// Returns through ctl->err (0 ok, ORX_NO_PATH, <0 error).
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rdst, bool full, float delta,
+                         uint64_t* keys, uint32_t* bmn0, uint32_t* bmn1, uint32_t* bmf, uint32_t* qA,
+                         uint32_t* qB, uint32_t* fA, uint32_t* fB, Ctl* ctl) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t R = B.R;
+    const uint32_t nwords = (R + 31) / 32;
+    for (uint32_t r = tid; r < R; r += BLOCK) keys[r] = ORX_KEY_INF;
+    for (uint32_t i = tid; i < nwords; i += BLOCK) { bmn0[i] = 0; bmn1[i] = 0; bmf[i] = 0; }
+    if (tid == 0) {
+        ctl->qcnt[0] = 1; ctl->qcnt[1] = 0; ctl->qcnt[2] = 0;
+        ctl->fcnt = 0; ctl->fcnt2 = 0; ctl->err = 0;
+        ctl->rounds = 0; ctl->phases = 0; ctl->items = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        keys[rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
+        qA[0] = rsrc;
+        bit_set(bmn0, rsrc);
+    }
+    __syncthreads();
+
+    float T = delta;  // d(src) = 0
+    uint32_t r = 0;   // round counter
+    for (;;) {
+        // ---------------- near phase: relax until the bucket [.., T) drains
+        for (;;) {
+            const uint32_t n = ctl->qcnt[r % 3];
+            if (n == 0) break;
+            uint32_t* cur = (r & 1) ? qB : qA;
+            uint32_t* nxt = (r & 1) ? qA : qB;
+            uint32_t* bmc = (r & 1) ? bmn1 : bmn0;
+            uint32_t* bmx = (r & 1) ? bmn0 : bmn1;
+            uint32_t* cnx = &ctl->qcnt[(r + 1) % 3];
+            if (tid == 0) { ctl->qcnt[(r + 2) % 3] = 0; ctl->rounds++; ctl->items += n; }
+            for (uint32_t base = 0; base < n; base += BLOCK) {
+                const uint32_t i = base + tid;
+                const bool active = i < n;
+                uint32_t u = 0;
+                uint64_t ku = ORX_KEY_INF;
+                float w4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+                uint32_t nb[4] = {UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
+                if (active) {
+                    u = cur[i];
+                    bit_clear(bmc, u);
+                    ku = key_load(keys + u);
+                    uint32_t lx, ly, z;
+                    roi_decode(B, u, lx, ly, z);
+                    const bool horiz = (g.hmask >> z) & 1u;
+                    const int64_t gu = roi_global(g, B, lx, ly, z);
+                    const float4 wv = reinterpret_cast<const float4*>(g.wcost)[gu];
+                    w4[0] = wv.x; w4[1] = wv.y; w4[2] = wv.z; w4[3] = wv.w;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) nb[s] = roi_nbr(B, horiz, u, lx, ly, z, s);
+                }
+                const float du = kdist(ku);
+                const uint32_t hu = khops(ku);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    bool push_near = false, push_far = false;
+                    const uint32_t v = nb[s];
+                    if (v != UINT32_MAX && !isinf(w4[s])) {
+                        const float nd = __fadd_rn(du, w4[s]);
+                        const uint64_t nk = kpack(nd, hu + 1, 3u - (uint32_t)s);
+                        if (nk < key_load(keys + v)) {
+                            const uint64_t old = key_min(keys + v, nk);
+                            if (nk < old) {
+                                if (nd < T) push_near = bit_set(bmx, v) == 0;
+                                else push_far = bit_set(bmf, v) == 0;
+                            }
+                        }
+                    }
+                    int at = wave_append(push_near, cnx);
+                    if (push_near) nxt[at] = v;
+                    at = wave_append(push_far, &ctl->fcnt);
+                    if (push_far) fA[at] = v;
+                }
+            }
+            ++r;
+            __syncthreads();
+        }
+        // ---------------- all keys below T are final
+        if (!full) {
+            const uint64_t kd = key_load(keys + rdst);
+            if (kd != ORX_KEY_INF && kdist(kd) < T) break;
+        }
+        const uint32_t nf = ctl->fcnt;
+        if (nf == 0) break;
+        // split pass 1: drop settled far entries, compact live ones, find min
+        if (tid == 0) { ctl->fcnt2 = 0; ctl->minbits = 0x7F800000u; ctl->phases++; }
+        __syncthreads();
+        float lmin = INFINITY;
+        for (uint32_t base = 0; base < nf; base += BLOCK) {
+            const uint32_t i = base + tid;
+            const bool active = i < nf;
+            bool keep = false;
+            uint32_t v = 0;
+            if (active) {
+                v = fA[i];
+                const float d = kdist(key_load(keys + v));
+                if (d < T) bit_clear(bmf, v);
+                else { keep = true; lmin = fminf(lmin, d); }
+            }
+            const int at = wave_append(keep, &ctl->fcnt2);
+            if (keep) fB[at] = v;
+        }
+        lmin = wave_min(lmin);
+        if (lane_id() == 0 && lmin < INFINITY) atomicMin(&ctl->minbits, __float_as_uint(lmin));
+        __syncthreads();
+        const uint32_t nf2 = ctl->fcnt2;
+        if (nf2 == 0) break;
+        const float mn = __uint_as_float(ctl->minbits);
+        float Tn = __fadd_rn(mn, delta);
+        if (!(Tn > mn)) Tn = nextafterf(mn, INFINITY);
+        if (tid == 0) ctl->fcnt = 0;
+        __syncthreads();
+        // split pass 2: entries below the new threshold form round r's queue
+        uint32_t* qr = (r & 1) ? qB : qA;
+        uint32_t* bmr = (r & 1) ? bmn1 : bmn0;
+        for (uint32_t base = 0; base < nf2; base += BLOCK) {
+            const uint32_t i = base + tid;
+            const bool active = i < nf2;
+            bool to_near = false, stay = false;
+            uint32_t v = 0;
+            if (active) {
+                v = fB[i];
+                const float d = kdist(key_load(keys + v));
+                if (d < Tn) { to_near = true; bit_clear(bmf, v); bit_set(bmr, v); }
+                else stay = true;
+            }
+            int at = wave_append(to_near, &ctl->qcnt[r % 3]);
+            if (to_near) qr[at] = v;
+            at = wave_append(stay, &ctl->fcnt);
+            if (stay) fA[at] = v;
+        }
+        T = Tn;
+        __syncthreads();
+    }
+    __syncthreads();
+}
+
+// Settled-set statistics of the finished search: the nodes a label-setting
+// search pops up to and including the sink (keys <= key(dst), or every
+// reached node without a sink) and the in-ROI out-degree of those popped
+// before the sink.
+template <int BLOCK>
+__device__ void box_stats(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rdst, bool full, Ctl* ctl) {
+    const uint64_t kd = full ? ORX_KEY_INF : key_load(keys + rdst);
+    unsigned long long settled = 0, relaxed = 0;
+    for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
+        const uint64_t k = key_load(keys + r);
+        if (k == ORX_KEY_INF) continue;
+        if (!full && k > kd) continue;
+        settled += 1;
+        if (!full && k == kd) continue;
+        uint32_t lx, ly, z;
+        roi_decode(B, r, lx, ly, z);
+        const bool horiz = (g.hmask >> z) & 1u;
+        for (int s = 0; s < 4; ++s) relaxed += roi_nbr(B, horiz, r, lx, ly, z, s) != UINT32_MAX;
+    }
+    settled = wave_sum(settled);
+    relaxed = wave_sum(relaxed);
+    if (lane_id() == 0) {
+        atomicAdd(&ctl->settled, settled);
+        atomicAdd(&ctl->relaxed, relaxed);
+    }
+}
+
+// Parent walk from the sink by key slots (single lane).  Writes len global
+// ids, source first.  Returns len or a negative status.
+__device__ int32_t box_backtrace(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rsrc, uint32_t rdst,
+                                 int32_t* path, int32_t cap) {
+    const uint64_t kd = key_load(keys + rdst);
+    if (kd == ORX_KEY_INF) return ORX_NO_PATH;
+    const uint32_t hops = khops(kd);
+    if ((int64_t)hops + 1 > cap) return ORX_E_OVERFLOW;
+    uint32_t v = rdst;
+    for (int32_t i = (int32_t)hops; i >= 0; --i) {
+        uint32_t lx, ly, z;
+        roi_decode(B, v, lx, ly, z);
+        path[i] = (int32_t)roi_global(g, B, lx, ly, z);
+        const uint32_t sl = kslot(key_load(keys + v));
+        if (i == 0) return (sl == ORX_ROOT_SLOT && v == rsrc) ? (int32_t)hops + 1 : ORX_E_CYCLE;
+        if (sl > 3) return ORX_E_CYCLE;
+        const bool horiz = (g.hmask >> z) & 1u;
+        const uint32_t p = roi_nbr(B, horiz, v, lx, ly, z, (int)sl);
+        if (p == UINT32_MAX) return ORX_E_CYCLE;
+        v = p;
+    }
+    return ORX_E_CYCLE;
+}
+
+__device__ __forceinline__ Box make_box(const int32_t* b, int32_t L) {
+    Box B;
+    B.x0 = b[0]; B.y0 = b[1]; B.w = b[2]; B.h = b[3]; B.L = L;
+    B.wh = (uint32_t)B.w * (uint32_t)B.h;
+    B.R = B.wh * (uint32_t)L;
+    B.dw = FastDiv((uint32_t)B.w);
+    B.dwh = FastDiv(B.wh);
+    return B;
+}
+
+__device__ __forceinline__ uint32_t global_to_roi(const Lat& g, const Box& B, int64_t gid) {
+    const int32_t z = (int32_t)(gid / g.XY);
+    const int64_t rem = gid - (int64_t)z * g.XY;
+    const int32_t y = (int32_t)(rem / g.X);
+    const int32_t x = (int32_t)(rem - (int64_t)y * g.X);
+    if (x < B.x0 || x >= B.x0 + B.w || y < B.y0 || y >= B.y0 + B.h) return UINT32_MAX;
+    return ((uint32_t)z * (uint32_t)B.h + (uint32_t)(y - B.y0)) * (uint32_t)B.w + (uint32_t)(x - B.x0);
+}
+
+// Search-state pointers of one wave slot.
+struct SlotState {
+    uint64_t* keys;
+    uint32_t *bmn0, *bmn1, *bmf, *qA, *qB, *fA, *fB;
+};
+
+__device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t off, bool lds, uint64_t* lds_keys,
+                                                int32_t lds_nodes) {
+    SlotState S;
+    S.qA = ws.queues + off;
+    S.qB = ws.queues + ws.ws_nodes + off;
+    S.fA = ws.queues + 2 * ws.ws_nodes + off;
+    S.fB = ws.queues + 3 * ws.ws_nodes + off;
+    if (lds) {
+        S.keys = lds_keys;
+        uint32_t* bm = reinterpret_cast<uint32_t*>(lds_keys + lds_nodes);
+        const int32_t words = (lds_nodes + 31) / 32;
+        S.bmn0 = bm; S.bmn1 = bm + words; S.bmf = bm + 2 * words;
+    } else {
+        S.keys = ws.keys + off;
+        S.bmn0 = ws.bitmaps + off / 32;
+        S.bmn1 = ws.bitmaps + ws.ws_nodes / 32 + off / 32;
+        S.bmf = ws.bitmaps + 2 * (ws.ws_nodes / 32) + off / 32;
+    }
+    return S;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, NetsDev nets, const int32_t* wave_nets,
+                                                           const int64_t* ws_off, orx_workspace ws, float delta,
+                                                           int32_t lds_nodes, int32_t flags,
+                                                           unsigned long long* stats) {
+    extern __shared__ uint64_t lds_keys[];
+    __shared__ Ctl ctl;
+    const int32_t k = wave_nets[blockIdx.x];
+    const Box B = make_box(nets.box + 4 * (int64_t)k, g.L);
+    const bool use_lds = (int64_t)B.R <= (int64_t)lds_nodes;
+    const SlotState S = slot_state(ws, ws_off[blockIdx.x], use_lds, lds_keys, lds_nodes);
+    int32_t* path = nets.path_arena + nets.path_off[k];
+    const int32_t cap = nets.path_cap[k];
+
+    if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; }
+    // 1. rip up the previous path and re-price what it held
+    const int32_t old_len = nets.path_len[k];
+    if (old_len > 0) {
+        apply_path_block<BLOCK>(g, path, old_len, -1);
+        __syncthreads();
+        reprice_path_block<BLOCK>(g, P, path, old_len);
+        __syncthreads();
+    }
+    // 2. search
+    const uint32_t rsrc = global_to_roi(g, B, nets.src[k]);
+    const uint32_t rdst = global_to_roi(g, B, nets.dst[k]);
+    if (rsrc == UINT32_MAX || rdst == UINT32_MAX) {
+        if (threadIdx.x == 0) { nets.status[k] = ORX_E_ARG; nets.path_len[k] = 0; }
+        return;
+    }
+    const bool full = (flags & ORX_F_FULL) != 0;
+    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB, &ctl);
+    if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, S.keys, rdst, full, &ctl);
+    // 3. backtrace
+    if (threadIdx.x == 0) {
+        int32_t len = box_backtrace(g, B, S.keys, rsrc, rdst, path, cap);
+        ctl.path_len = len;
+        nets.status[k] = len > 0 ? ORX_OK : len;
+        nets.path_len[k] = len > 0 ? len : 0;
+        nets.path_cost[k] = len > 0 ? kdist(key_load(S.keys + rdst)) : INFINITY;
+    }
+    __syncthreads();
+    const int32_t len = ctl.path_len;
+    // 4. commit and re-price
+    if (len > 0) {
+        apply_path_block<BLOCK>(g, path, len, +1);
+        __syncthreads();
+        reprice_path_block<BLOCK>(g, P, path, len);
+    }
+    if (stats && threadIdx.x == 0) {
+        atomicAdd(stats + 0, ctl.settled);
+        atomicAdd(stats + 1, ctl.relaxed);
+        atomicAdd(stats + 2, (unsigned long long)ctl.items);
+        atomicAdd(stats + 3, (unsigned long long)ctl.rounds);
+        atomicAdd(stats + 4, (unsigned long long)ctl.phases);
+        atomicAdd(stats + (len > 0 ? 5 : 6), 1ull);
+        if (len > 0) atomicAdd(stats + 7, (unsigned long long)(len - 1));
+    }
+}
+
+// Single box query writing ROI-local dist / parent arrays.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws, int32_t x0, int32_t y0, int32_t w,
+                                                         int32_t h, int64_t src, int64_t dst, float delta,
+                                                         int32_t lds_nodes, int32_t flags, float* dist,
+                                                         int32_t* parent, unsigned long long* stats,
+                                                         int32_t* status) {
+    extern __shared__ uint64_t lds_keys[];
+    __shared__ Ctl ctl;
+    const int32_t bx[4] = {x0, y0, w, h};
+    const Box B = make_box(bx, g.L);
+    const bool use_lds = (int64_t)B.R <= (int64_t)lds_nodes;
+    const SlotState S = slot_state(ws, 0, use_lds, lds_keys, lds_nodes);
+    if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; }
+    const uint32_t rsrc = global_to_roi(g, B, src);
+    const bool full = (flags & ORX_F_FULL) != 0 || dst < 0;
+    const uint32_t rdst = dst >= 0 ? global_to_roi(g, B, dst) : 0;
+    if (rsrc == UINT32_MAX || rdst == UINT32_MAX) {
+        if (threadIdx.x == 0) *status = ORX_E_ARG;
+        return;
+    }
+    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB, &ctl);
+    if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, S.keys, rdst, full, &ctl);
+    for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
+        const uint64_t k = key_load(S.keys + r);
+        if (dist) dist[r] = k == ORX_KEY_INF ? INFINITY : kdist(k);
+        if (parent) {
+            int32_t p = -1;
+            if (k != ORX_KEY_INF && kslot(k) != ORX_ROOT_SLOT) {
+                uint32_t lx, ly, z;
+                roi_decode(B, r, lx, ly, z);
+                const bool horiz = (g.hmask >> z) & 1u;
+                const uint32_t pr = roi_nbr(B, horiz, r, lx, ly, z, (int)kslot(k));
+                if (pr != UINT32_MAX) {
+                    uint32_t plx, ply, pz;
+                    roi_decode(B, pr, plx, ply, pz);
+                    p = (int32_t)roi_global(g, B, plx, ply, pz);
+                }
+            }
+            parent[r] = p;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t kd = key_load(S.keys + rdst);
+        *status = (!full && kd == ORX_KEY_INF) ? ORX_NO_PATH : ORX_OK;
+        if (stats) {
+            atomicAdd(stats + 0, ctl.settled);
+            atomicAdd(stats + 1, ctl.relaxed);
+            atomicAdd(stats + 2, (unsigned long long)ctl.items);
+            atomicAdd(stats + 3, (unsigned long long)ctl.rounds);
+            atomicAdd(stats + 4, (unsigned long long)ctl.phases);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Whole-lattice streaming kernels (HBM-bound)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void reprice_all_kernel(Lat g, Params P, int64_t N) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < N; u += (int64_t)gridDim.x * blockDim.x) {
+        const float4 b4 = reinterpret_cast<const float4*>(g.base)[u];
+        const float4 h4 = reinterpret_cast<const float4*>(g.hist)[u];
+        const int4 o4 = reinterpret_cast<const int4*>(g.occ)[u];
+        const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+        const float hh[4] = {h4.x, h4.y, h4.z, h4.w};
+        const int32_t oo[4] = {o4.x, o4.y, o4.z, o4.w};
+        float ww[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            ww[s] = INFINITY;
+            if (isinf(bb[s])) continue;
+            const int64_t v = lat_nbr(g, u, s);
+            if (v < 0) continue;
+            const float q = node_cost(g.node_hist[v], g.node_use[v], P);
+            ww[s] = __fadd_rn(arc_cost(bb[s], hh[s], oo[s], P), q);
+        }
+        reinterpret_cast<float4*>(g.wcost)[u] = make_float4(ww[0], ww[1], ww[2], ww[3]);
+    }
+}
+
+// Overuse + history, fused.  out: edge_over, edge_over_count, node_over,
+// node_over_count, via_hops, barrels.
+__global__ __launch_bounds__(256) void end_iteration_kernel(Lat g, Params P, int64_t N, unsigned long long* out) {
+    unsigned long long eo = 0, ec = 0, no = 0, nc = 0, vh = 0, br = 0;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < N; u += (int64_t)gridDim.x * blockDim.x) {
+        const float4 b4 = reinterpret_cast<const float4*>(g.base)[u];
+        float4 h4 = reinterpret_cast<const float4*>(g.hist)[u];
+        const int4 o4 = reinterpret_cast<const int4*>(g.occ)[u];
+        const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+        float hh[4] = {h4.x, h4.y, h4.z, h4.w};
+        const int32_t oo[4] = {o4.x, o4.y, o4.z, o4.w};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            int32_t over = oo[s] - P.edge_cap;
+            over = over < 0 ? 0 : over;
+            if (s >= 2 && over > 0) { eo += over; ec += 1; }
+            float inc = __fmul_rn(P.hist_gain, (float)over);
+            if (P.hist_cap_mult > 0.0f) {
+                const float capv = __fmul_rn(P.hist_cap_mult, bb[s]);
+                if (capv < inc) inc = capv;
+            }
+            hh[s] = __fadd_rn(__fmul_rn(hh[s], P.hist_decay), inc);
+        }
+        vh += (unsigned long long)oo[3];
+        reinterpret_cast<float4*>(g.hist)[u] = make_float4(hh[0], hh[1], hh[2], hh[3]);
+        int32_t on = g.node_use[u] - P.node_cap;
+        on = on < 0 ? 0 : on;
+        if (on > 0) { no += on; nc += 1; }
+        g.node_hist[u] = __fadd_rn(__fmul_rn(g.node_hist[u], P.hist_decay), __fmul_rn(P.node_hist_gain, (float)on));
+        if (u < g.XY) br += (unsigned long long)g.via_col[u];
+    }
+    eo = wave_sum(eo); ec = wave_sum(ec); no = wave_sum(no); nc = wave_sum(nc); vh = wave_sum(vh); br = wave_sum(br);
+    if (lane_id() == 0) {
+        if (eo) atomicAdd(out + 0, eo);
+        if (ec) atomicAdd(out + 1, ec);
+        if (no) atomicAdd(out + 2, no);
+        if (nc) atomicAdd(out + 3, nc);
+        if (vh) atomicAdd(out + 4, vh);
+        if (br) atomicAdd(out + 5, br);
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_i64_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void apply_paths_kernel(Lat g, const int32_t* paths, const int64_t* off,
+                                                            const int32_t* len, int32_t sign) {
+    apply_path_block<BLOCK>(g, paths + off[blockIdx.x], len[blockIdx.x], sign);
+}
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void reprice_paths_kernel(Lat g, Params P, const int32_t* paths,
+                                                              const int64_t* off, const int32_t* len) {
+    reprice_path_block<BLOCK>(g, P, paths + off[blockIdx.x], len[blockIdx.x]);
+}
+
+// Per-net overuse over the forward arcs of its committed path: the
+// numerator of _order_nets_by_difficulty's congestion term
+// (unified_pathfinder.py:7858-7863; both arcs of a hop hold equal occupancy).
+__global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, int32_t edge_cap, int64_t* out_over,
+                                                          int32_t* out_hops) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nets.n) return;
+    const int32_t len = nets.path_len[k];
+    const int32_t* path = nets.path_arena + nets.path_off[k];
+    int64_t over = 0;
+    for (int32_t i = 0; i + 1 < len; ++i) {
+        const int64_t u = path[i];
+        const int s = lat_slot(g, u, path[i + 1]);
+        const int32_t o = g.occ[4 * u + s] - edge_cap;
+        over += o > 0 ? o : 0;
+    }
+    out_over[k] = over;
+    out_hops[k] = len > 0 ? len - 1 : 0;
+}
+
+Lat make_lat(const orx_lattice* l) {
+    Lat g;
+    g.X = l->X; g.Y = l->Y; g.L = l->L; g.hmask = l->hmask;
+    g.XY = (int64_t)l->X * l->Y;
+    g.base = l->base; g.hist = l->hist; g.occ = l->occ; g.wcost = l->wcost;
+    g.node_hist = l->node_hist; g.node_use = l->node_use; g.via_col = l->via_col;
+    return g;
+}
+Params make_params(const orx_params* p) {
+    Params P;
+    P.pres_fac = p->pres_fac; P.base_w = p->base_w; P.hist_w = p->hist_w; P.node_pres = p->node_pres;
+    P.node_hist_w = p->node_hist_w; P.edge_cap = p->edge_cap; P.node_cap = p->node_cap;
+    P.hist_gain = p->hist_gain; P.hist_decay = p->hist_decay; P.hist_cap_mult = p->hist_cap_mult;
+    P.node_hist_gain = p->node_hist_gain;
+    return P;
+}
+
+int check_lattice(const orx_lattice* l) {
+    if (!l) return orx_host::fail(ORX_E_ARG, "lattice is NULL");
+    if (l->X < 1 || l->Y < 1 || l->L < 1 || l->L > 32)
+        return orx_host::fail(ORX_E_ARG, "lattice dims must be X,Y >= 1 and 1 <= L <= 32");
+    if ((int64_t)l->X * l->Y * l->L >= (1ll << 31))
+        return orx_host::fail(ORX_E_ARG, "lattice too large for int32 node ids");
+    if (!l->base || !l->hist || !l->occ || !l->wcost || !l->node_hist || !l->node_use || !l->via_col)
+        return orx_host::fail(ORX_E_ARG, "lattice has a NULL array");
+    if (((uintptr_t)l->base | (uintptr_t)l->hist | (uintptr_t)l->occ | (uintptr_t)l->wcost) & 15u)
+        return orx_host::fail(ORX_E_ARG, "arc arrays must be 16-byte aligned");
+    return ORX_OK;
+}
+
+size_t lds_bytes_for(int32_t lds_nodes) {
+    return (size_t)lds_nodes * 8 + 3 * (size_t)((lds_nodes + 31) / 32) * 4;
+}
+
+template <typename K>
+int set_lds(K kernel, size_t bytes) {
+    static size_t configured[8] = {0};
+    (void)configured;
+    ORX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return ORX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orx_lattice_reprice(const orx_lattice* lat, const orx_params* p, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!p) return orx_host::fail(ORX_E_ARG, "params is NULL");
+    const int64_t N = (int64_t)lat->X * lat->Y * lat->L;
+    const int grid = (int)std::min<int64_t>((N + 255) / 256, 65536);
+    hipLaunchKernelGGL(reprice_all_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, make_lat(lat),
+                       make_params(p), N);
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+int orx_lattice_end_iteration(const orx_lattice* lat, const orx_params* p, int64_t* d_out, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!p || !d_out) return orx_host::fail(ORX_E_ARG, "params / out is NULL");
+    const int64_t N = (int64_t)lat->X * lat->Y * lat->L;
+    hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (unsigned long long*)d_out, 8);
+    const int grid = (int)std::min<int64_t>((N + 255) / 256, 8192);
+    hipLaunchKernelGGL(end_iteration_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, make_lat(lat),
+                       make_params(p), N, (unsigned long long*)d_out);
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* nets, const int32_t* d_wave_nets,
+                   const int64_t* d_ws_off, int32_t K, int64_t max_nodes, const orx_workspace* ws,
+                   const orx_route_cfg* cfg, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!p || !nets || !ws || !cfg || !d_wave_nets || !d_ws_off) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (K <= 0) return ORX_OK;
+    if (max_nodes >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI larger than 2^24 nodes");
+    if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
+    int32_t lds_nodes = cfg->lds_nodes;
+    if (lds_nodes < 0) lds_nodes = 0;
+    if ((int64_t)lds_nodes > max_nodes) lds_nodes = (int32_t)((max_nodes + 63) / 64 * 64);
+    const size_t lds = lds_bytes_for(lds_nodes);
+    if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
+    NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
+               nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+    unsigned long long* st = (unsigned long long*)cfg->stats;
+    const hipStream_t s = (hipStream_t)stream;
+    const Lat g = make_lat(lat);
+    const Params P = make_params(p);
+#define ORX_LAUNCH_ROUTE(BS)                                                                              \
+    do {                                                                                                  \
+        rc = set_lds(route_wave_kernel<BS>, lds);                                                         \
+        if (rc) return rc;                                                                                \
+        hipLaunchKernelGGL(route_wave_kernel<BS>, dim3(K), dim3(BS), lds, s, g, P, nd, d_wave_nets,       \
+                           d_ws_off, *ws, cfg->delta, lds_nodes, cfg->flags, st);                         \
+    } while (0)
+    switch (cfg->block) {
+    case 256: ORX_LAUNCH_ROUTE(256); break;
+    case 512: ORX_LAUNCH_ROUTE(512); break;
+    case 1024: ORX_LAUNCH_ROUTE(1024); break;
+    default: return orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
+    }
+#undef ORX_LAUNCH_ROUTE
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_cap, int64_t* d_over,
+                    int32_t* d_hops, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!nets || !d_over || !d_hops) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (nets->n <= 0) return ORX_OK;
+    NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
+               nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+    hipLaunchKernelGGL(net_overuse_kernel, dim3((nets->n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       make_lat(lat), nd, edge_cap, d_over, d_hops);
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+int orx_apply_paths(const orx_lattice* lat, const orx_params* p, const int32_t* d_paths, const int64_t* d_off,
+                    const int32_t* d_len, int32_t K, int32_t sign, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!p || !d_paths || !d_off || !d_len) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (sign != 1 && sign != -1) return orx_host::fail(ORX_E_ARG, "sign must be +1 or -1");
+    if (K <= 0) return ORX_OK;
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(apply_paths_kernel<256>, dim3(K), dim3(256), 0, s, make_lat(lat), d_paths, d_off, d_len, sign);
+    hipLaunchKernelGGL(reprice_paths_kernel<256>, dim3(K), dim3(256), 0, s, make_lat(lat), make_params(p), d_paths,
+                       d_off, d_len);
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int32_t h, int64_t src, int64_t dst,
+                 float* d_dist, int32_t* d_parent, const orx_workspace* ws, const orx_route_cfg* cfg,
+                 orx_stream_t stream) {
+    if (!lat || !lat->wcost) return orx_host::fail(ORX_E_ARG, "lattice / wcost is NULL");
+    if (!ws || !cfg) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (x0 < 0 || y0 < 0 || w < 1 || h < 1 || x0 + w > lat->X || y0 + h > lat->Y)
+        return orx_host::fail(ORX_E_ARG, "box outside the lattice");
+    const int64_t R = (int64_t)w * h * lat->L;
+    if (R >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI larger than 2^24 nodes");
+    if (ws->ws_nodes < R) return orx_host::fail(ORX_E_ARG, "workspace smaller than the ROI");
+    int32_t lds_nodes = cfg->lds_nodes < 0 ? 0 : cfg->lds_nodes;
+    if ((int64_t)lds_nodes > R) lds_nodes = (int32_t)((R + 63) / 64 * 64);
+    const size_t lds = lds_bytes_for(lds_nodes);
+    if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
+    int32_t* d_status = nullptr;
+    int32_t h_status = 0;
+    ORX_HIP_CHECK(hipMallocAsync((void**)&d_status, sizeof(int32_t), (hipStream_t)stream));
+    const hipStream_t s = (hipStream_t)stream;
+    Lat g;
+    g.X = lat->X; g.Y = lat->Y; g.L = lat->L; g.hmask = lat->hmask; g.XY = (int64_t)lat->X * lat->Y;
+    g.base = lat->base; g.hist = lat->hist; g.occ = lat->occ; g.wcost = lat->wcost;
+    g.node_hist = lat->node_hist; g.node_use = lat->node_use; g.via_col = lat->via_col;
+    int rc = ORX_OK;
+#define ORX_LAUNCH_BOX(BS)                                                                                 \
+    do {                                                                                                   \
+        rc = set_lds(sssp_box_kernel<BS>, lds);                                                            \
+        if (rc) break;                                                                                     \
+        hipLaunchKernelGGL(sssp_box_kernel<BS>, dim3(1), dim3(BS), lds, s, g, *ws, x0, y0, w, h, src, dst, \
+                           cfg->delta, lds_nodes, cfg->flags, d_dist, d_parent,                            \
+                           (unsigned long long*)cfg->stats, d_status);                                     \
+    } while (0)
+    switch (cfg->block) {
+    case 256: ORX_LAUNCH_BOX(256); break;
+    case 512: ORX_LAUNCH_BOX(512); break;
+    case 1024: ORX_LAUNCH_BOX(1024); break;
+    default: rc = orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
+    }
+#undef ORX_LAUNCH_BOX
+    if (rc == ORX_OK) {
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&h_status, d_status, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = orx_host::check_hip(e, "orx_sssp_box");
+        else rc = h_status;
+    }
+    (void)hipFreeAsync(d_status, s);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+"""Synthetic routing workloads on the 3D lattice.
+
+Each generator is seeded and deterministic (NumPy PCG64).  Pins sit on layer
+0 (the pin/escape layer) and are pairwise distinct.  Every net gets an ROI
+box: its pins' bounding box plus a margin, clipped to the board, spanning all
+layers.  Workloads:
+
+* ``plain_64``  -- 64x64x6, 200 two-pin nets, pins uniform without
+  replacement on layer 0, seed 0;
+* ``backplane_400`` -- 400x400x16, two 40x40 pin arrays at opposite ends
+  joined by a seeded permutation (1600 long nets) plus 400 short nets with
+  Manhattan pin spacing uniform in 2..20;
+* ``uneven_800`` -- 800x800x20, 8000 two-pin nets, Manhattan length
+  lognormal (median 25, sigma 1, clipped to 2..700), sources uniform,
+  directions random, seed 3, ROI margin 4;
+* ``serving_job`` -- one synthetic board of the serving stream.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .lattice import LatticeSpec
+
+
+@dataclass
+class Netlist:
+    spec: LatticeSpec
+    src: np.ndarray   # int32 global node ids (layer 0)
+    dst: np.ndarray
+    box: np.ndarray   # int32 [n, 4]: x0, y0, w, h
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return len(self.src)
+
+    def roi_nodes(self) -> np.ndarray:
+        return self.box[:, 2].astype(np.int64) * self.box[:, 3] * self.spec.L
+
+    def subset(self, k: int) -> "Netlist":
+        return Netlist(self.spec, self.src[:k].copy(), self.dst[:k].copy(), self.box[:k].copy(),
+                       f"{self.name}[:{k}]", dict(self.meta))
+
+
+def roi_boxes(spec: LatticeSpec, src: np.ndarray, dst: np.ndarray, margin: int) -> np.ndarray:
+    """Bounding box of the two pins plus ``margin`` cells, clipped to the board."""
+    sx, sy, _ = spec.coords(src)
+    tx, ty, _ = spec.coords(dst)
+    x0 = np.maximum(np.minimum(sx, tx) - margin, 0)
+    y0 = np.maximum(np.minimum(sy, ty) - margin, 0)
+    x1 = np.minimum(np.maximum(sx, tx) + margin, spec.X - 1)
+    y1 = np.minimum(np.maximum(sy, ty) + margin, spec.Y - 1)
+    return np.stack([x0, y0, x1 - x0 + 1, y1 - y0 + 1], axis=1).astype(np.int32)
+
+
+def _place_two_pin(rng, spec: LatticeSpec, lengths: np.ndarray, used: np.ndarray,
+                   forbid: np.ndarray | None = None, max_tries: int = 64):
+    """For each net: a random Manhattan split (|dx|, |dy|) of its length and
+    random signs, then a source uniform over the layer-0 cells from which that
+    offset stays on the board.  Resamples when a pin cell is already used;
+    after ``max_tries`` failures (a length near the board diagonal leaves only
+    corner placements) the length shrinks by 10% and sampling continues."""
+    X, Y = spec.X, spec.Y
+    n = len(lengths)
+    src = np.empty(n, np.int64)
+    dst = np.empty(n, np.int64)
+    for i in range(n):
+        ell = int(min(max(lengths[i], 1), (X - 1) + (Y - 1)))
+        for attempt in range(16 * max_tries):
+            if attempt and attempt % max_tries == 0:
+                ell = max(1, int(ell * 0.9))
+            ax = min(int(round(ell * float(rng.random()))), X - 1)
+            ay = ell - ax
+            if ay > Y - 1:
+                ay = Y - 1
+                ax = ell - ay
+            gx = 1 if rng.random() < 0.5 else -1
+            gy = 1 if rng.random() < 0.5 else -1
+            sx = int(rng.integers(0, X - ax)) + (ax if gx < 0 else 0)
+            sy = int(rng.integers(0, Y - ay)) + (ay if gy < 0 else 0)
+            tx, ty = sx + gx * ax, sy + gy * ay
+            s, t = sy * X + sx, ty * X + tx
+            if s == t or used[s] or used[t]:
+                continue
+            if forbid is not None and (forbid[s] or forbid[t]):
+                continue
+            used[s] = True
+            used[t] = True
+            src[i], dst[i] = s, t
+            break
+        else:
+            raise RuntimeError("could not place a net; board too crowded")
+    return src, dst
+
+
+def plain_64(seed: int = 0, margin: int = 4) -> Netlist:
+    """64x64x6, 200 two-pin nets with pins uniform without replacement on layer 0."""
+    spec = LatticeSpec(64, 64, 6)
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(spec.XY, size=400, replace=False).astype(np.int64)
+    src, dst = cells[0::2], cells[1::2]
+    return Netlist(spec, src.astype(np.int32), dst.astype(np.int32), roi_boxes(spec, src, dst, margin),
+                   "plain_64x64x6_200", {"seed": seed, "margin": margin})
+
+
+def lognormal_lengths(rng, n: int, median: float = 25.0, sigma: float = 1.0, lo: int = 2, hi: int = 700):
+    ell = rng.lognormal(mean=np.log(median), sigma=sigma, size=n)
+    return np.clip(np.rint(ell), lo, hi).astype(np.int64)
+
+
+def uneven(X: int, Y: int, L: int, n_nets: int, seed: int, margin: int = 4, name: str = "") -> Netlist:
+    spec = LatticeSpec(X, Y, L)
+    rng = np.random.default_rng(seed)
+    lengths = lognormal_lengths(rng, n_nets)
+    used = np.zeros(spec.XY, bool)
+    src, dst = _place_two_pin(rng, spec, lengths, used)
+    return Netlist(spec, src.astype(np.int32), dst.astype(np.int32), roi_boxes(spec, src, dst, margin),
+                   name or f"uneven_{X}x{Y}x{L}_{n_nets}", {"seed": seed, "margin": margin})
+
+
+def uneven_800(seed: int = 3, n_nets: int = 8000, margin: int = 4) -> Netlist:
+    """800x800x20 with 8000 lognormal-length two-pin nets (seed 3)."""
+    return uneven(800, 800, 20, n_nets, seed, margin, "uneven_800x800x20_8000")
+
+
+def backplane_400(seed: int = 2, margin: int = 4) -> Netlist:
+    """400x400x16: two 40x40 pin arrays at opposite ends joined by a seeded
+    permutation (1600 long nets) plus 400 short nets, spacing U{2..20}."""
+    spec = LatticeSpec(400, 400, 16)
+    rng = np.random.default_rng(seed)
+    ax0, bx0, y0 = 20, 340, 180
+    gx, gy = np.meshgrid(np.arange(40), np.arange(40), indexing="xy")
+    a = ((y0 + gy) * spec.X + (ax0 + gx)).reshape(-1)
+    b = ((y0 + gy) * spec.X + (bx0 + gx)).reshape(-1)
+    perm = rng.permutation(1600)
+    long_src, long_dst = a.astype(np.int64), b[perm].astype(np.int64)
+    used = np.zeros(spec.XY, bool)
+    used[a] = True
+    used[b] = True
+    spacing = rng.integers(2, 21, size=400)
+    s_src, s_dst = _place_two_pin(rng, spec, spacing, used)
+    src = np.concatenate([long_src, s_src])
+    dst = np.concatenate([long_dst, s_dst])
+    return Netlist(spec, src.astype(np.int32), dst.astype(np.int32), roi_boxes(spec, src, dst, margin),
+                   "backplane_400x400x16_2000", {"seed": seed, "margin": margin})
+
+
+SERVING_SIDES = (128, 256, 512, 1024)
+SERVING_LAYERS = (8, 16, 24, 32)
+
+
+def serving_job(rng, n_min: int = 500, n_max: int = 10000, margin: int = 4) -> Netlist:
+    """One board of the serving stream: side in {128,256,512,1024}, layers in
+    {8,16,24,32}, 500..10000 nets generated like uneven_800."""
+    side = int(rng.choice(SERVING_SIDES))
+    L = int(rng.choice(SERVING_LAYERS))
+    n = int(rng.integers(n_min, n_max + 1))
+    n = min(n, (side * side) // 4)
+    seed = int(rng.integers(0, 2**31 - 1))
+    return uneven(side, side, L, n, seed, margin, f"job_{side}x{side}x{L}_{n}")

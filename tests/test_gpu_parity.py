@@ -1,0 +1,99 @@
+"""GPU parity: the HIP path against the CPU oracle, bit-exact.
+
+Integer and index results (paths, occupancy, node use, via columns, overuse,
+settled / relaxed counts) must be identical; fp32 results (path costs,
+history, distances) must be bitwise identical too, because both sides apply
+the same fp32 operations in the same order.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.oracle import OraclePathFinder, sssp_box as oracle_sssp_box
+from orthoroute_amd.netgen import plain_64, uneven, backplane_400
+from orthoroute_amd.router import DevicePathFinder, LatticeState, RouterConfig
+from orthoroute_amd.lattice import LatticeSpec
+from orthoroute_amd.sssp import sssp_box
+
+pytestmark = pytest.mark.gpu
+
+STAT_KEYS = ["routed", "failed", "edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops",
+             "barrels", "settled", "relaxed", "path_hops"]
+
+
+def _assert_same_iteration(sd, so):
+    got = {k: getattr(sd, k) for k in STAT_KEYS}
+    want = {k: so[k] for k in STAT_KEYS}
+    assert got == want, f"iteration {so['iteration']}: {got} != {want}"
+
+
+def _assert_same_state(dev: DevicePathFinder, orc: OraclePathFinder):
+    st = dev.lat.to_numpy()
+    for k in ("occ", "node_use", "via_col"):
+        np.testing.assert_array_equal(st[k], getattr(orc, k), err_msg=k)
+    for k in ("hist", "node_hist"):
+        assert np.array_equal(st[k].view(np.uint32), getattr(orc, k).view(np.uint32)), k
+    dp, op = dev.paths(), orc.paths()
+    assert len(dp) == len(op)
+    for i, (a, b) in enumerate(zip(dp, op)):
+        assert np.array_equal(a, b), f"net {i}: path differs"
+    dc = dev.path_costs()
+    assert np.array_equal(dc.view(np.uint32), orc.path_cost.view(np.uint32))
+
+
+def _run_both(nl, cfg, iters, mode, device):
+    dev = DevicePathFinder(nl, cfg, device=device)
+    orc = OraclePathFinder(nl, cfg)
+    for _ in range(iters):
+        print(f"[parity] {nl.name} mode={mode} iteration {dev.iteration + 1}", flush=True)
+        sd = dev.run_iteration(mode)
+        so = orc.run_iteration()
+        _assert_same_iteration(sd, so)
+        _assert_same_state(dev, orc)
+    return dev, orc
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_plain_64_ten_iterations_bit_exact(device, mode):
+    nl = plain_64()
+    cfg = RouterConfig(iterations=10)
+    dev, orc = _run_both(nl, cfg, 10, mode, device)
+    assert dev.history[-1].overuse == orc.history[-1]["edge_over"] + orc.history[-1]["node_over"]
+
+
+@pytest.mark.parametrize("lds_nodes", [0, 12288])
+@pytest.mark.parametrize("delta", [1.0, 3.0, float("inf")])
+def test_uneven_small_bit_exact(device, lds_nodes, delta):
+    nl = uneven(96, 80, 8, 300, seed=11)
+    cfg = RouterConfig(iterations=4, delta=delta, lds_nodes=lds_nodes)
+    _run_both(nl, cfg, 4, "batched", device)
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_block_sizes_bit_exact(device, block):
+    nl = uneven(64, 64, 6, 150, seed=5)
+    cfg = RouterConfig(iterations=3, block=block)
+    _run_both(nl, cfg, 3, "batched", device)
+
+
+def test_backplane_subset_bit_exact(device):
+    nl = backplane_400()
+    sub = nl.subset(200)
+    cfg = RouterConfig(iterations=2)
+    _run_both(sub, cfg, 2, "batched", device)
+
+
+@pytest.mark.parametrize("lds_nodes", [0, 12288])
+def test_sssp_box_random_weights(device, lds_nodes):
+    spec = LatticeSpec(20, 17, 5)
+    rng = np.random.default_rng(7)
+    w = rng.uniform(1.0, 4.0, 4 * spec.N).astype(np.float32)
+    w[spec.neighbours().reshape(-1) < 0] = np.inf
+    lat = LatticeState(spec, device=device)
+    lat.wcost.copy_(torch.from_numpy(w))
+    box = (2, 1, 15, 14)
+    for src in (spec.node(5, 5, 0), spec.node(10, 9, 3)):
+        dist, parent, rc = sssp_box(lat, box, int(src), full=True, lds_nodes=lds_nodes)
+        od, op, _, _, _ = oracle_sssp_box(spec, w, box, int(src))
+        assert np.array_equal(dist.cpu().numpy().view(np.uint32), od.view(np.uint32))
+        np.testing.assert_array_equal(parent.cpu().numpy(), op)
