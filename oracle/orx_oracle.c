@@ -130,6 +130,12 @@ static inline int lat_slot(const lat_t* g, int64_t u, int64_t v) {
 typedef struct { uint64_t k; int32_t v; } hent;
 typedef struct { hent* a; int64_t n, cap; } heap_t;
 
+/* Pop order (key, node): equal packed keys (same distance, hops and slot) pop
+ * by node id, so "the nodes settled up to the sink" is canonical. */
+static inline int hless(uint64_t ka, int32_t va, uint64_t kb, int32_t vb) {
+    return ka < kb || (ka == kb && va < vb);
+}
+
 static int heap_push(heap_t* h, uint64_t k, int32_t v) {
     if (h->n == h->cap) {
         int64_t nc = h->cap ? h->cap * 2 : 1024;
@@ -140,7 +146,7 @@ static int heap_push(heap_t* h, uint64_t k, int32_t v) {
     int64_t i = h->n++;
     while (i > 0) {
         int64_t p = (i - 1) >> 1;
-        if (h->a[p].k <= k) break;
+        if (!hless(k, v, h->a[p].k, h->a[p].v)) break;
         h->a[i] = h->a[p];
         i = p;
     }
@@ -154,8 +160,8 @@ static hent heap_pop(heap_t* h) {
     for (;;) {
         int64_t l = 2 * i + 1;
         if (l >= n) break;
-        int64_t c = (l + 1 < n && h->a[l + 1].k < h->a[l].k) ? l + 1 : l;
-        if (h->a[c].k >= last.k) break;
+        int64_t c = (l + 1 < n && hless(h->a[l + 1].k, h->a[l + 1].v, h->a[l].k, h->a[l].v)) ? l + 1 : l;
+        if (!hless(h->a[c].k, h->a[c].v, last.k, last.v)) break;
         h->a[i] = h->a[c];
         i = c;
     }

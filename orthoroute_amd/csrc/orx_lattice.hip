@@ -297,9 +297,9 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
 }
 
 // Settled-set statistics of the finished search: the nodes a label-setting
-// search pops up to and including the sink (keys <= key(dst), or every
-// reached node without a sink) and the in-ROI out-degree of those popped
-// before the sink.
+// search popping in (key, node id) order settles up to and including the
+// sink (every reached node without a sink), and the in-ROI out-degree of
+// those popped before the sink.  ROI-local indices order like global ids.
 template <int BLOCK>
 __device__ void box_stats(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rdst, bool full, Ctl* ctl) {
     const uint64_t kd = full ? ORX_KEY_INF : key_load(keys + rdst);
@@ -307,9 +307,9 @@ __device__ void box_stats(const Lat& g, const Box& B, const uint64_t* keys, uint
     for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
         const uint64_t k = key_load(keys + r);
         if (k == ORX_KEY_INF) continue;
-        if (!full && k > kd) continue;
+        if (!full && (k > kd || (k == kd && r > rdst))) continue;
         settled += 1;
-        if (!full && k == kd) continue;
+        if (!full && r == rdst) continue;
         uint32_t lx, ly, z;
         roi_decode(B, r, lx, ly, z);
         const bool horiz = (g.hmask >> z) & 1u;

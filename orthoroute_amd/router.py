@@ -91,6 +91,7 @@ class IterationStats:
     waves: int = 0
     seconds: float = 0.0
     gpu_ms: float = 0.0
+    route_ms: float = 0.0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -300,9 +301,12 @@ class DevicePathFinder:
         self._route_stats.zero_()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
+        evr0 = torch.cuda.Event(enable_timing=True)
+        evr1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         self.lat.reprice(params, stream)
         rcfg = self.route_cfg()
+        evr0.record(stream)
         s = self._s()
         pos = 0
         for w in waves:
@@ -312,6 +316,7 @@ class DevicePathFinder:
                       d_nets.data_ptr() + 4 * pos, d_off.data_ptr() + 8 * pos, K, maxR,
                       C.byref(self.ws.struct), C.byref(rcfg), s)
             pos += K
+        evr1.record(stream)
         _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
                   self._iter_out.data_ptr(), s)
         ev1.record(stream)
@@ -324,7 +329,8 @@ class DevicePathFinder:
             edge_over=int(out[0]), edge_over_count=int(out[1]), node_over=int(out[2]),
             node_over_count=int(out[3]), via_hops=int(out[4]), barrels=int(out[5]),
             settled=int(rs[0]), relaxed=int(rs[1]), items=int(rs[2]), rounds=int(rs[3]), phases=int(rs[4]),
-            path_hops=int(rs[7]), waves=len(waves), seconds=dt, gpu_ms=ev0.elapsed_time(ev1))
+            path_hops=int(rs[7]), waves=len(waves), seconds=dt, gpu_ms=ev0.elapsed_time(ev1),
+            route_ms=evr0.elapsed_time(evr1))
         self.iteration = it
         self.history.append(st)
         return st

@@ -1,0 +1,218 @@
+"""Drop-in accelerator for the engine's solver seam.
+
+``HipDijkstra`` exposes the methods the reference engine calls on its
+``gpu_solver`` slot and on ``SimpleDijkstra``
+(unified_pathfinder.py:2451-2659, 3099-3120, 8083-8093):
+
+* ``find_path_roi_gpu`` / ``find_path_roi`` (cuda_dijkstra.py:4951-5040):
+  source at distance 0, ROI-local node penalty paid when a node is entered;
+* ``find_path_multisource_multisink`` (unified_pathfinder.py:2550-2659):
+  seeds pay their own penalty, targets are sinks, returns
+  (path, entry_layer, exit_layer);
+* ``find_path_fullgraph_gpu_seeds`` (cuda_dijkstra.py:5516-5537): whole graph,
+  seed / target costs, global node penalty, optional allowed bitmap.
+
+All return global node ids (source first) or ``None`` when no path exists,
+like the reference.  ``try_attach_hip`` grafts the accelerator onto a
+SimpleDijkstra the way try_attach_metal does (metal_dijkstra.py:44-89): the
+wrapped methods call the HIP solver first and hand a ``None`` result to the
+solver's own original method, so a net the caller could route is never
+dropped.  Inside this package there is no CPU search: when liborx_hip.so or
+the GPU is missing the constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import logging
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .lattice import in_rank_and_transpose
+
+log = logging.getLogger(__name__)
+
+
+def _host(a) -> np.ndarray:
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    if hasattr(a, "get"):
+        return a.get()
+    return np.asarray(a)
+
+
+class HipDijkstra:
+    """SSSP over a CSR graph on the GPU (generic form of the lattice kernels)."""
+
+    def __init__(self, graph=None, lattice=None, *, indptr=None, indices=None, device="cuda",
+                 delta: float = 1.0, block: int = 256):
+        _lib.init_device()
+        if graph is not None:
+            indptr = graph.indptr
+            indices = graph.indices
+        ip = _host(indptr).astype(np.int64)
+        ix = _host(indices).astype(np.int64)
+        self.N = len(ip) - 1
+        self.E = len(ix)
+        if self.N >= 2**24:
+            raise ValueError("graphs above 2^24 nodes are not supported by the packed key")
+        rank, in_indptr, in_src = in_rank_and_transpose(self.N, ip, ix)
+        self.device = torch.device(device)
+        dev = self.device
+        self.indptr = torch.from_numpy(ip.astype(np.int32)).to(dev)
+        self.indices = torch.from_numpy(ix.astype(np.int32)).to(dev)
+        self.in_rank = torch.from_numpy(rank).to(dev)
+        self.in_indptr = torch.from_numpy(in_indptr).to(dev)
+        self.in_src = torch.from_numpy(in_src).to(dev)
+        self.graph = _lib.OrxCsr(self.N, self.E, self.indptr.data_ptr(), self.indices.data_ptr(),
+                                 self.in_rank.data_ptr(), self.in_indptr.data_ptr(), self.in_src.data_ptr())
+        self.plane_size = (lattice.x_steps * lattice.y_steps) if lattice is not None and hasattr(lattice, "x_steps") \
+            else None
+        self.delta = float(delta)
+        self.block = int(block)
+        self._cap = 0
+        self._key = self._scratch = self._path = None
+        self._small = torch.zeros(4, dtype=torch.int32, device=dev)  # best, len
+
+    # -- buffers -------------------------------------------------------------
+    def _ensure(self, R: int):
+        if R <= self._cap:
+            return
+        R = int(R)
+        words = (R + 31) // 32
+        self._key = torch.empty(R, dtype=torch.int64, device=self.device)
+        self._scratch = torch.empty(4 * R + 4 * words, dtype=torch.int32, device=self.device)
+        self._path = torch.empty(R, dtype=torch.int32, device=self.device)
+        self._cap = R
+
+    def _dev(self, a, dtype) -> Optional[torch.Tensor]:
+        if a is None:
+            return None
+        if isinstance(a, torch.Tensor):
+            return a.to(self.device, dtype).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(_host(a)).astype(
+            {torch.float32: np.float32, torch.int32: np.int32}[dtype])).to(self.device)
+
+    # -- core ----------------------------------------------------------------
+    def _search(self, costs, R, roi_nodes, g2r, pen, allowed, seeds, seed_cost, targets, tg_cost,
+                seed_pen: bool) -> Optional[List[int]]:
+        self._ensure(R)
+        d_costs = self._dev(costs, torch.float32)
+        if d_costs.numel() != self.E:
+            raise ValueError(f"costs has {d_costs.numel()} entries, graph has {self.E} arcs")
+        d_seeds = self._dev(np.asarray(seeds, np.int32), torch.int32)
+        d_sc = self._dev(seed_cost, torch.float32) if seed_cost is not None else None
+        d_tg = self._dev(np.asarray(targets, np.int32), torch.int32)
+        d_tc = self._dev(tg_cost, torch.float32) if tg_cost is not None else None
+        d_roi = self._dev(roi_nodes, torch.int32) if roi_nodes is not None else None
+        d_g2r = self._dev(g2r, torch.int32) if g2r is not None else None
+        d_pen = self._dev(pen, torch.float32) if pen is not None else None
+        d_allowed = None
+        if allowed is not None:
+            d_allowed = torch.from_numpy(np.ascontiguousarray(_host(allowed)).astype(np.uint32).view(np.int32)).to(
+                self.device)
+        flags = _lib.ORX_F_SEED_PEN if seed_pen else 0
+        cfg = _lib.OrxRouteCfg(self.delta, self.block, 0, flags, None)
+        stream = torch.cuda.current_stream().cuda_stream
+        words = (R + 31) // 32
+        best = self._small[0:1]
+        plen = self._small[1:2]
+        p = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        _lib.call("orx_sssp_csr", C.byref(self.graph), d_costs.data_ptr(), int(R), p(d_roi), p(d_g2r), p(d_pen),
+                  p(d_allowed), len(seeds), d_seeds.data_ptr(), p(d_sc), len(targets), d_tg.data_ptr(), p(d_tc),
+                  self._key.data_ptr(), self._scratch.data_ptr(), 4 * R + 4 * words, best.data_ptr(),
+                  C.byref(cfg), stream)
+        _lib.call("orx_backtrace_csr", C.byref(self.graph), self._key.data_ptr(), p(d_g2r), best.data_ptr(),
+                  self._path.data_ptr(), int(self._cap), plen.data_ptr(), stream)
+        n = int(plen.item())
+        if n == _lib.ORX_NO_PATH:
+            return None
+        if n < 0:
+            raise _lib.HipError("orx_backtrace_csr", n, "parent walk failed")
+        path = self._path[:n].cpu().numpy().astype(np.int64).tolist()
+        return path if len(path) > 1 else None
+
+    # -- seam methods --------------------------------------------------------
+    def find_path_roi_gpu(self, src: int, dst: int, costs, roi_nodes, global_to_roi,
+                          node_penalty=None) -> Optional[List[int]]:
+        g2r = _host(global_to_roi)
+        if g2r[int(src)] < 0 or g2r[int(dst)] < 0:
+            return None
+        R = len(roi_nodes)
+        return self._search(costs, R, roi_nodes, g2r, node_penalty, None, [int(src)], None, [int(dst)], None,
+                            seed_pen=False)
+
+    find_path_roi = find_path_roi_gpu
+
+    def find_path_multisource_multisink(self, src_seeds: Sequence[Tuple[int, float]],
+                                        dst_targets: Sequence[Tuple[int, float]], costs, roi_nodes,
+                                        global_to_roi, node_penalty=None):
+        g2r = _host(global_to_roi)
+        seeds = [(int(n), float(c)) for n, c in src_seeds if g2r[int(n)] >= 0]
+        tgs = [(int(n), float(c)) for n, c in dst_targets if g2r[int(n)] >= 0]
+        if not seeds or not tgs:
+            return None
+        path = self._search(costs, len(roi_nodes), roi_nodes, g2r, node_penalty, None,
+                            [n for n, _ in seeds], np.asarray([c for _, c in seeds], np.float32),
+                            [n for n, _ in tgs], np.asarray([c for _, c in tgs], np.float32), seed_pen=True)
+        if path is None:
+            return None
+        if self.plane_size:
+            return path, path[0] // self.plane_size, path[-1] // self.plane_size
+        return path, 0, 0
+
+    def find_path_fullgraph_gpu_seeds(self, costs, src_seeds, dst_targets, ub_hint=None, *,
+                                      src_seed_costs=None, dst_target_costs=None, node_penalty=None,
+                                      allowed_bitmap=None, use_bitmap=False) -> Optional[List[int]]:
+        seeds = np.asarray(_host(src_seeds), np.int32).reshape(-1)
+        tgs = np.asarray(_host(dst_targets), np.int32).reshape(-1)
+        if seeds.size == 0 or tgs.size == 0:
+            return None
+        sc = np.zeros(seeds.size, np.float32) if src_seed_costs is None else \
+            np.asarray(_host(src_seed_costs), np.float32).reshape(-1)
+        tc = np.zeros(tgs.size, np.float32) if dst_target_costs is None else \
+            np.asarray(_host(dst_target_costs), np.float32).reshape(-1)
+        allowed = allowed_bitmap if use_bitmap else None
+        return self._search(costs, self.N, None, None, node_penalty, allowed, seeds.tolist(), sc, tgs.tolist(), tc,
+                            seed_pen=True)
+
+
+def try_attach_hip(solver, graph, lattice=None, *, device="cuda") -> bool:
+    """Graft the HIP solver onto a SimpleDijkstra-like object (the
+    ORTHO_BACKEND=hip counterpart of try_attach_metal, metal_dijkstra.py:44-89).
+
+    Sets ``solver.gpu_solver`` (unified_pathfinder.py:3099-3120) and wraps
+    ``find_path_roi`` / ``find_path_multisource_multisink`` so a ``None``
+    from the HIP solver falls through to the solver's own original method.
+    Returns False (solver untouched) when the library or GPU is unavailable.
+    """
+    try:
+        hip = HipDijkstra(graph, lattice, device=device)
+    except (_lib.HipUnavailable, RuntimeError, OSError) as e:
+        log.info("[HIP] accelerator not attached: %s", e)
+        return False
+    orig_roi = solver.find_path_roi
+    orig_multi = getattr(solver, "find_path_multisource_multisink", None)
+
+    def roi(src, dst, costs, roi_nodes, global_to_roi, node_penalty=None):
+        path = hip.find_path_roi_gpu(src, dst, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+        if path is not None:
+            return path
+        return orig_roi(src, dst, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+
+    def multi(src_seeds, dst_targets, costs, roi_nodes, global_to_roi, node_penalty=None):
+        res = hip.find_path_multisource_multisink(src_seeds, dst_targets, costs, roi_nodes, global_to_roi,
+                                                  node_penalty=node_penalty)
+        if res is not None or orig_multi is None:
+            return res
+        return orig_multi(src_seeds, dst_targets, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+
+    solver.find_path_roi = roi
+    if orig_multi is not None:
+        solver.find_path_multisource_multisink = multi
+    solver.gpu_solver = hip
+    solver.hip_solver = hip
+    log.info("[HIP] gfx950 accelerator attached")
+    return True
