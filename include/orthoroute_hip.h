@@ -73,6 +73,13 @@ typedef struct orx_params {
     float hist_decay;
     float hist_cap_mult;   /* > 0: increment <= mult*base (UP:1255-1262)   */
     float node_hist_gain;
+    /* A* toward the sink in orx_route_wave (0 = plain Dijkstra order):
+     * per-move lower bounds of the entering weight, planar / via, each at
+     * least half an fp32 ulp of astar_gmax below the true minimum; results
+     * with a sink distance >= astar_gmax are recomputed without it. */
+    float astar_cp;
+    float astar_cv;
+    float astar_gmax;
 } orx_params;
 
 /* Per-net tables of one routing job (resident in HBM). */
@@ -148,6 +155,31 @@ int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* 
                    const int32_t* d_wave_nets, const int64_t* d_ws_off, int32_t K,
                    int64_t max_nodes, const orx_workspace* ws, const orx_route_cfg* cfg,
                    orx_stream_t stream);
+
+/* Route every net of one iteration in ONE persistent launch.  d_sched[n]
+ * is a topological order of the nets; net k may start once every net in
+ * d_dep_ids[d_dep_ptr[k] .. d_dep_ptr[k+1]) has published d_done[id] == epoch
+ * (the last earlier-ordered nets covering a cell of its ROI box, from
+ * orx_plan_iteration).  This reproduces the sequential per-net loop of
+ * _route_all (UP:7903-8449) exactly, with disjoint-ROI nets in flight
+ * together.  d_ws_off[k]: per-net workspace offset (multiple of 64);
+ * d_counter: one u32 the call zeroes; max_groups <= 0: one group per CU slot
+ * the occupancy allows. */
+int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* nets,
+                  const int32_t* d_sched, int32_t n, const int32_t* d_dep_ptr,
+                  const int32_t* d_dep_ids, int32_t* d_done, int32_t epoch, uint32_t* d_counter,
+                  const int64_t* d_ws_off, int64_t max_nodes, const orx_workspace* ws,
+                  const orx_route_cfg* cfg, int32_t max_groups, orx_stream_t stream);
+
+/* Host planner of one iteration (no device work): disjoint-ROI wave of every
+ * net, a schedule (by wave, larger ROI first) and the dependency lists of
+ * orx_route_dag.  Inputs: box[4n], order[n] (routing order), board X x Y,
+ * roi_nodes[n].  Outputs: wave[n], sched[n], dep_ptr[n+1], dep_ids[dep_cap].
+ * Returns the number of dependencies (> dep_cap: nothing written to dep_ids,
+ * call again with a larger buffer) or < 0 on bad input. */
+int64_t orx_plan_iteration(const int32_t* box, const int32_t* order, int32_t n, int32_t X,
+                           int32_t Y, const int64_t* roi_nodes, int32_t* wave, int32_t* sched,
+                           int32_t* dep_ptr, int32_t* dep_ids, int64_t dep_cap);
 
 /* Per-net overuse of its committed path, summed over the forward arcs
  * (d_over[n]) and its hop count (d_hops[n]): the congestion input of

@@ -43,6 +43,7 @@ class OracleParams(C.Structure):
         ("edge_cap", C.c_int32), ("node_cap", C.c_int32),
         ("hist_gain", C.c_float), ("hist_decay", C.c_float),
         ("hist_cap_mult", C.c_float), ("node_hist_gain", C.c_float),
+        ("astar_cp", C.c_float), ("astar_cv", C.c_float), ("astar_gmax", C.c_float),
     ]
 
 
@@ -130,6 +131,7 @@ class OraclePathFinder:
         self.dst = netlist.dst.astype(np.int32)
         self.distance = net_distance(spec, netlist.src, netlist.dst)
         self.schedule = cfg.schedule(spec.L)
+        self.astar = cfg.astar_bounds(self.base)
         self.iteration = 0
         self.history = []
 
@@ -161,6 +163,7 @@ class OraclePathFinder:
         stats = np.zeros(16, np.int64)
         P = params_from(self.cfg, 1.0)
         P.node_pres = float(np.float32(self.cfg.node_pres_w))  # scaled by pres inside
+        P.astar_cp, P.astar_cv, P.astar_gmax = self.astar
         rc = self.lib.orx_oracle_pathfinder(
             s.X, s.Y, s.L, s.hmask, _p(self.base), self.netlist.n, _p(self.src), _p(self.dst), _p(self.box),
             1, _p(order), _p(pres), C.byref(P), _p(self.hist), _p(self.occ), _p(self.node_hist),

@@ -69,6 +69,9 @@ typedef struct {
     float hist_decay;      /* history *= decay before the increment          */
     float hist_cap_mult;   /* increment <= mult*base when > 0                 */
     float node_hist_gain;  /* node-history increment per unit of overuse     */
+    float astar_cp;        /* A* planar step bound (0 = plain Dijkstra)      */
+    float astar_cv;        /* A* via step bound                              */
+    float astar_gmax;      /* A* result trusted only below this distance     */
 } orx_params;
 
 /* c_arc = fl(fl(fl(b*bw) + fl(hw*h)) + fl(p*max(0, occ+1-cap)))
@@ -125,18 +128,19 @@ static inline int lat_slot(const lat_t* g, int64_t u, int64_t v) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Binary min-heap of (key, node) with lazy deletion                        */
+/* Binary min-heap of (f, key, node) with lazy deletion                     */
 /* ------------------------------------------------------------------------ */
-typedef struct { uint64_t k; int32_t v; } hent;
+typedef struct { double f; uint64_t k; int32_t v; } hent;
 typedef struct { hent* a; int64_t n, cap; } heap_t;
 
-/* Pop order (key, node): equal packed keys (same distance, hops and slot) pop
- * by node id, so "the nodes settled up to the sink" is canonical. */
-static inline int hless(uint64_t ka, int32_t va, uint64_t kb, int32_t vb) {
-    return ka < kb || (ka == kb && va < vb);
+/* Pop order (f, key, node), f = g + heuristic (f = g for plain Dijkstra):
+ * equal packed keys pop by node id, so "the nodes settled up to the sink"
+ * is canonical. */
+static inline int hless(double fa, uint64_t ka, int32_t va, double fb, uint64_t kb, int32_t vb) {
+    return fa < fb || (fa == fb && (ka < kb || (ka == kb && va < vb)));
 }
 
-static int heap_push(heap_t* h, uint64_t k, int32_t v) {
+static int heap_push(heap_t* h, double f, uint64_t k, int32_t v) {
     if (h->n == h->cap) {
         int64_t nc = h->cap ? h->cap * 2 : 1024;
         hent* na = (hent*)realloc(h->a, (size_t)nc * sizeof(hent));
@@ -146,11 +150,11 @@ static int heap_push(heap_t* h, uint64_t k, int32_t v) {
     int64_t i = h->n++;
     while (i > 0) {
         int64_t p = (i - 1) >> 1;
-        if (!hless(k, v, h->a[p].k, h->a[p].v)) break;
+        if (!hless(f, k, v, h->a[p].f, h->a[p].k, h->a[p].v)) break;
         h->a[i] = h->a[p];
         i = p;
     }
-    h->a[i].k = k; h->a[i].v = v;
+    h->a[i].f = f; h->a[i].k = k; h->a[i].v = v;
     return 0;
 }
 static hent heap_pop(heap_t* h) {
@@ -160,8 +164,9 @@ static hent heap_pop(heap_t* h) {
     for (;;) {
         int64_t l = 2 * i + 1;
         if (l >= n) break;
-        int64_t c = (l + 1 < n && hless(h->a[l + 1].k, h->a[l + 1].v, h->a[l].k, h->a[l].v)) ? l + 1 : l;
-        if (!hless(h->a[c].k, h->a[c].v, last.k, last.v)) break;
+        int64_t c = (l + 1 < n && hless(h->a[l + 1].f, h->a[l + 1].k, h->a[l + 1].v,
+                                        h->a[l].f, h->a[l].k, h->a[l].v)) ? l + 1 : l;
+        if (!hless(h->a[c].f, h->a[c].k, h->a[c].v, last.f, last.k, last.v)) break;
         h->a[i] = h->a[c];
         i = c;
     }
@@ -217,12 +222,34 @@ static int ws_reserve(sssp_ws* W, int64_t R) {
 }
 static void ws_free(sssp_ws* W) { free(W->key); free(W->done); free(W->heap.a); memset(W, 0, sizeof(*W)); }
 
+/* A* heuristic toward the sink (tx, ty, tz):
+ *   h(v) = cp*(|x-tx| + |y-ty|) + cv*|z-tz|, evaluated exactly in double.
+ * cp / cv are per-move lower bounds of the entering weight minus a margin
+ * m >= half an fp32 ulp below gmax, so for every tight arc u->v with
+ * g(v) < gmax, f(u) = g(u)+h(u) <= g(v)+h(v) = f(v): popping by f settles
+ * exactly the canonical keys.  A result with g(dst) >= gmax is recomputed
+ * without the heuristic (box_route). */
+typedef struct { int on; double cp, cv; int32_t tx, ty, tz; } astar_t;
+
+static inline double heur(const astar_t* A, const lat_t* g, int64_t gid) {
+    if (!A || !A->on) return 0.0;
+    int64_t XY = (int64_t)g->X * g->Y;
+    int32_t z = (int32_t)(gid / XY);
+    int64_t rem = gid - (int64_t)z * XY;
+    int32_t y = (int32_t)(rem / g->X), x = (int32_t)(rem - (int64_t)y * g->X);
+    int32_t dx = x > A->tx ? x - A->tx : A->tx - x;
+    int32_t dy = y > A->ty ? y - A->ty : A->ty - y;
+    int32_t dz = z > A->tz ? z - A->tz : A->tz - z;
+    return A->cp * (double)(dx + dy) + A->cv * (double)dz;
+}
+
 /* Returns 0 when dst was reached (or full sweep), 1 when unreachable, <0 on error.
  * stats[0] += settled nodes, stats[1] += relaxed arcs (out-arcs of settled
  * nodes inside the ROI, each arc once). */
 static int box_sssp(const lat_t* g, const lat_state* S, const orx_params* P,
                     int32_t x0, int32_t y0, int32_t w, int32_t h,
-                    int64_t src, int64_t dst, int full, sssp_ws* W, int64_t* stats) {
+                    int64_t src, int64_t dst, int full, sssp_ws* W, int64_t* stats,
+                    const astar_t* A) {
     const int64_t XY = (int64_t)g->X * g->Y;
     const int64_t R = (int64_t)w * h * g->L;
     if (ws_reserve(W, R)) return -2;
@@ -239,7 +266,7 @@ static int box_sssp(const lat_t* g, const lat_state* S, const orx_params* P,
     int64_t rd = dst >= 0 ? G2R(dst) : -1;
     if (rs < 0 || (dst >= 0 && rd < 0)) return -3;
     W->key[rs] = kpack(0.0f, 0, ROOT_SLOT);
-    if (heap_push(&W->heap, W->key[rs], (int32_t)rs)) return -2;
+    if (heap_push(&W->heap, heur(A, g, src), W->key[rs], (int32_t)rs)) return -2;
     while (W->heap.n) {
         hent e = heap_pop(&W->heap);
         if (W->done[e.v] || e.k != W->key[e.v]) continue;
@@ -263,7 +290,7 @@ static int box_sssp(const lat_t* g, const lat_state* S, const orx_params* P,
             uint64_t nk = kpack(nd, hu + 1, (uint32_t)(3 - s));
             if (nk < W->key[rv]) {
                 W->key[rv] = nk;
-                if (heap_push(&W->heap, nk, (int32_t)rv)) return -2;
+                if (heap_push(&W->heap, (double)nd + heur(A, g, gv), nk, (int32_t)rv)) return -2;
             }
         }
     }
@@ -271,6 +298,31 @@ static int box_sssp(const lat_t* g, const lat_state* S, const orx_params* P,
     return 0;
 #undef G2R
 #undef R2G
+}
+
+/* Sink-directed search of the router: A* when P enables it, recomputed
+ * without the heuristic when the result is not below astar_gmax.  Stats are
+ * those of the search whose keys are kept. */
+static int box_route(const lat_t* g, const lat_state* S, const orx_params* P,
+                     int32_t x0, int32_t y0, int32_t w, int32_t h,
+                     int64_t src, int64_t dst, sssp_ws* W, int64_t* stats) {
+    int64_t st[2] = {0, 0};
+    if (P->astar_cp > 0.0f && P->astar_cv > 0.0f && P->astar_gmax > 0.0f) {
+        const int64_t XY = (int64_t)g->X * g->Y;
+        astar_t A;
+        A.on = 1; A.cp = P->astar_cp; A.cv = P->astar_cv;
+        A.tz = (int32_t)(dst / XY);
+        int64_t rem = dst - (int64_t)A.tz * XY;
+        A.ty = (int32_t)(rem / g->X); A.tx = (int32_t)(rem - (int64_t)A.ty * g->X);
+        int r = box_sssp(g, S, P, x0, y0, w, h, src, dst, 0, W, st, &A);
+        if (r < 0 || r == 1) { stats[0] += st[0]; stats[1] += st[1]; return r; }
+        uint64_t kd = W->key[box_index(g, x0, y0, w, h, dst)];
+        if (kdist(kd) < P->astar_gmax) { stats[0] += st[0]; stats[1] += st[1]; return r; }
+        st[0] = st[1] = 0;
+    }
+    int r = box_sssp(g, S, P, x0, y0, w, h, src, dst, 0, W, st, NULL);
+    stats[0] += st[0]; stats[1] += st[1];
+    return r;
 }
 
 /* Walk parents from dst; writes hops+1 global ids into path (src first).
@@ -448,7 +500,7 @@ int orx_oracle_pathfinder(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const
                 path_len[k] = 0;
             }
             const int32_t* b = box + 4 * (int64_t)k;
-            int r = box_sssp(&g, &S, &P, b[0], b[1], b[2], b[3], src[k], dst[k], 0, &W, st + 8);
+            int r = box_route(&g, &S, &P, b[0], b[1], b[2], b[3], src[k], dst[k], &W, st + 8);
             if (r < 0) { rc = r; goto done; }
             if (r == 1) { st[1] += 1; continue; }
             int64_t len = box_backtrace(&g, b[0], b[1], b[2], b[3], dst[k], &W, pk, path_cap[k]);
@@ -484,7 +536,7 @@ int orx_oracle_sssp_box(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const f
     orx_params P; memset(&P, 0, sizeof(P));
     sssp_ws W; memset(&W, 0, sizeof(W));
     stats[0] = stats[1] = 0;
-    int r = box_sssp(&g, &S, &P, x0, y0, w, h, src, full ? -1 : dst, full, &W, stats);
+    int r = box_sssp(&g, &S, &P, x0, y0, w, h, src, full ? -1 : dst, full, &W, stats, NULL);
     if (r >= 0) {
         int64_t R = (int64_t)w * h * L, XY = (int64_t)X * Y, wh = (int64_t)w * h;
         for (int64_t i = 0; i < R; ++i) {
@@ -552,7 +604,7 @@ int orx_oracle_sssp_csr(int64_t N, const int32_t* indptr, const int32_t* indices
         float c = seed_cost ? seed_cost[i] : 0.0f;
         if (pen && seed_pen) c = c + pen[r];
         uint64_t k = kpack(c, 0, ROOT_SLOT);
-        if (k < key[r]) { key[r] = k; if (heap_push(&H, k, (int32_t)r)) { rc = -2; goto out; } }
+        if (k < key[r]) { key[r] = k; if (heap_push(&H, (double)c, k, (int32_t)r)) { rc = -2; goto out; } }
     }
     int64_t remaining = 0;
     for (int64_t r = 0; r < R; ++r) remaining += is_tg[r];
@@ -580,7 +632,7 @@ int orx_oracle_sssp_csr(int64_t N, const int32_t* indptr, const int32_t* indices
             float nd = du + wt;
             if (hu + 1 > HOPS_MAX || in_rank[a] > 254) { rc = -4; goto out; }
             uint64_t nk = kpack(nd, hu + 1, (uint32_t)in_rank[a]);
-            if (nk < key[rv]) { key[rv] = nk; if (heap_push(&H, nk, (int32_t)rv)) { rc = -2; goto out; } }
+            if (nk < key[rv]) { key[rv] = nk; if (heap_push(&H, (double)nd, nk, (int32_t)rv)) { rc = -2; goto out; } }
         }
     }
     {

@@ -51,6 +51,7 @@ class OrxParams(C.Structure):
         ("edge_cap", C.c_int32), ("node_cap", C.c_int32),
         ("hist_gain", C.c_float), ("hist_decay", C.c_float),
         ("hist_cap_mult", C.c_float), ("node_hist_gain", C.c_float),
+        ("astar_cp", C.c_float), ("astar_cv", C.c_float), ("astar_gmax", C.c_float),
     ]
 
 
@@ -88,6 +89,10 @@ SIGNATURES = [
     ("orx_lattice_end_iteration", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), P, P]),
     ("orx_route_wave", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
                                  C.c_int32, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
+    ("orx_route_dag", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, C.c_int32, P, P,
+                                P, C.c_int32, P, P, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg),
+                                C.c_int32, P]),
+    ("orx_plan_iteration", C.c_int64, [P, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int64]),
     ("orx_net_overuse", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxNets), C.c_int32, P, P, P]),
     ("orx_apply_paths", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), P, P, P, C.c_int32,
                                   C.c_int32, P]),

@@ -60,7 +60,7 @@ def tune(nl, deltas=(1.0, 2.0, 4.0, 8.0, float("inf")), blocks=(256, 512, 1024),
         order = d.order_for(1)
         waves = d.plan(order, "batched")
         pick = list(range(min(sample_waves, len(waves)))) + list(range(sample_waves, len(waves), stride))
-        params = cfg.params(float(d.schedule[0]))
+        params = cfg.params(float(d.schedule[0]), d.astar)
         d.lat.reprice(params)
         rcfg = d.route_cfg()
         s = torch.cuda.current_stream()

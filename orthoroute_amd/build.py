@@ -24,7 +24,8 @@ BUILD_DIR = REPO_DIR / "_build"
 LIB_PATH = PKG_DIR / "liborx_hip.so"
 ORACLE_DIR = REPO_DIR / "oracle"
 
-SOURCES = ["orx_capi.hip", "orx_lattice.hip", "orx_csr.hip", "orx_grid.hip"]
+SOURCES = ["orx_capi.hip", "orx_lattice.hip", "orx_csr.hip", "orx_grid.hip", "orx_plan.cpp"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -62,7 +63,10 @@ def build_hip(verbose: bool = False) -> Path:
 
     def compile_one(pair):
         src, obj = pair
-        cmd = [hipcc, *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+        if src.suffix == ".cpp":  # host-only planner
+            cmd = ["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)]
+        else:
+            cmd = [hipcc, *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)

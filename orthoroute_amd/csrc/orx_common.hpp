@@ -15,6 +15,17 @@
 
 #pragma clang fp contract(off)
 
+// Dynamic LDS of a kernel.  The host-thread test build of the kernels
+// (tests/cpu_build) predefines it to per-workgroup host memory.
+#ifndef ORX_DYNAMIC_LDS
+#define ORX_DYNAMIC_LDS(T, name) extern __shared__ T name[]
+#endif
+// One workgroup-shared variable of a kernel (same test build: per-workgroup
+// host storage, so workgroups can run concurrently there).
+#ifndef ORX_SHARED
+#define ORX_SHARED(T, name) __shared__ T name
+#endif
+
 #define ORX_KEY_INF 0xFFFFFFFFFFFFFFFFull
 #define ORX_ROOT_SLOT 0xFFu
 #define ORX_HOPS_MAX 0xFFFFFFu
@@ -73,6 +84,7 @@ struct Params {
     float pres_fac, base_w, hist_w, node_pres, node_hist_w;
     int32_t edge_cap, node_cap;
     float hist_gain, hist_decay, hist_cap_mult, node_hist_gain;
+    float astar_cp, astar_cv, astar_gmax;
 };
 
 __device__ __forceinline__ float arc_cost(float b, float h, int32_t occ, const Params& P) {

@@ -75,7 +75,7 @@ __device__ __forceinline__ bool allowed_node(const CsrArgs& A, int64_t gid) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void csr_sssp_kernel(CsrArgs A) {
-    __shared__ CsrCtl ctl;
+    ORX_SHARED(CsrCtl, ctl);
     const uint32_t tid = threadIdx.x;
     const uint32_t R = (uint32_t)A.R;
     const uint32_t words = (R + 31) / 32;

@@ -20,6 +20,8 @@
 // T <- min(far) + delta.  With early exit the search stops once the sink's
 // distance is below T (all keys below T are final).
 #include "orx_common.hpp"
+
+#include <algorithm>
 #include "../../include/orthoroute_hip.h"
 
 using namespace orx;
@@ -43,7 +45,7 @@ struct Ctl {
     uint32_t qcnt[3];
     uint32_t fcnt;
     uint32_t fcnt2;
-    uint32_t minbits;
+    unsigned long long minf;
     int32_t err;
     int32_t path_len;
     uint32_t rounds, phases, items;
@@ -151,13 +153,52 @@ __device__ void reprice_path_block(const Lat& g, const Params& P, const int32_t*
     for (int32_t i = threadIdx.x; i < len; i += BLOCK) reprice_in_arcs(g, P, path[i]);
 }
 
+// A* heuristic toward the sink (global coords tx, ty, tz):
+//   h(v) = cp*(|x-tx| + |y-ty|) + cv*|z-tz|, evaluated exactly in double.
+// cp / cv are per-move lower bounds of the entering weight reduced by a
+// margin of at least half an fp32 ulp below gmax, so along every tight arc
+// u->v with g(v) < gmax, f(u) = g(u) + h(u) <= f(v): bucketing by f settles
+// exactly the canonical keys of the sink and its parent chain.  A result with
+// g(dst) >= gmax is recomputed without the heuristic by the caller.
+struct Heur {
+    bool on;
+    double cp, cv;
+    int32_t tx, ty, tz;
+    __device__ __forceinline__ double h(int32_t x, int32_t y, int32_t z) const {
+        if (!on) return 0.0;
+        const int32_t dx = x > tx ? x - tx : tx - x;
+        const int32_t dy = y > ty ? y - ty : ty - y;
+        const int32_t dz = z > tz ? z - tz : tz - z;
+        return cp * (double)(dx + dy) + cv * (double)dz;
+    }
+};
+
+__device__ __forceinline__ double box_h(const Heur& H, const Box& B, uint32_t lx, uint32_t ly, uint32_t z) {
+    return H.h(B.x0 + (int32_t)lx, B.y0 + (int32_t)ly, (int32_t)z);
+}
+
+// Heuristic of the neighbour of (lx, ly, z) in slot s.
+__device__ __forceinline__ double nbr_h(const Heur& H, const Box& B, bool horiz, uint32_t lx, uint32_t ly, uint32_t z,
+                                       int s) {
+    int32_t x = B.x0 + (int32_t)lx, y = B.y0 + (int32_t)ly, zz = (int32_t)z;
+    switch (s) {
+    case 0: zz -= 1; break;
+    case 1: if (horiz) x -= 1; else y -= 1; break;
+    case 2: if (horiz) x += 1; else y += 1; break;
+    default: zz += 1; break;
+    }
+    return H.h(x, y, zz);
+}
+
+__device__ __forceinline__ unsigned long long dbits(double d) { return (unsigned long long)__double_as_longlong(d); }
+
 // ---------------------------------------------------------------------------
-// Near-far delta-stepping over one ROI box, one workgroup.
-// Returns through ctl->err (0 ok, ORX_NO_PATH, <0 error).
+// Near-far delta-stepping over one ROI box, one workgroup, bucketed by
+// f = g + h (h = 0 without a heuristic).
 // ---------------------------------------------------------------------------
 template <int BLOCK>
 __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rdst, bool full, float delta,
-                         uint64_t* keys, uint32_t* bmn0, uint32_t* bmn1, uint32_t* bmf, uint32_t* qA,
+                         const Heur& H, uint64_t* keys, uint32_t* bmn0, uint32_t* bmn1, uint32_t* bmf, uint32_t* qA,
                          uint32_t* qB, uint32_t* fA, uint32_t* fB, Ctl* ctl) {
     const uint32_t tid = threadIdx.x;
     const uint32_t R = B.R;
@@ -167,9 +208,16 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
     if (tid == 0) {
         ctl->qcnt[0] = 1; ctl->qcnt[1] = 0; ctl->qcnt[2] = 0;
         ctl->fcnt = 0; ctl->fcnt2 = 0; ctl->err = 0;
-        ctl->rounds = 0; ctl->phases = 0; ctl->items = 0;
     }
     __syncthreads();
+    double T;
+    {
+        uint32_t lx, ly, z;
+        roi_decode(B, rsrc, lx, ly, z);
+        const double f0 = box_h(H, B, lx, ly, z);
+        T = f0 + (double)delta;
+        if (!(T > f0)) T = nextafter(f0, (double)INFINITY);
+    }
     if (tid == 0) {
         keys[rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
         qA[0] = rsrc;
@@ -177,7 +225,6 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
     }
     __syncthreads();
 
-    float T = delta;  // d(src) = 0
     uint32_t r = 0;   // round counter
     for (;;) {
         // ---------------- near phase: relax until the bucket [.., T) drains
@@ -193,7 +240,8 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
             for (uint32_t base = 0; base < n; base += BLOCK) {
                 const uint32_t i = base + tid;
                 const bool active = i < n;
-                uint32_t u = 0;
+                uint32_t u = 0, lx = 0, ly = 0, z = 0;
+                bool horiz = false;
                 uint64_t ku = ORX_KEY_INF;
                 float w4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
                 uint32_t nb[4] = {UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
@@ -201,9 +249,8 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
                     u = cur[i];
                     bit_clear(bmc, u);
                     ku = key_load(keys + u);
-                    uint32_t lx, ly, z;
                     roi_decode(B, u, lx, ly, z);
-                    const bool horiz = (g.hmask >> z) & 1u;
+                    horiz = (g.hmask >> z) & 1u;
                     const int64_t gu = roi_global(g, B, lx, ly, z);
                     const float4 wv = reinterpret_cast<const float4*>(g.wcost)[gu];
                     w4[0] = wv.x; w4[1] = wv.y; w4[2] = wv.z; w4[3] = wv.w;
@@ -222,7 +269,8 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
                         if (nk < key_load(keys + v)) {
                             const uint64_t old = key_min(keys + v, nk);
                             if (nk < old) {
-                                if (nd < T) push_near = bit_set(bmx, v) == 0;
+                                const double nf = (double)nd + nbr_h(H, B, horiz, lx, ly, z, s);
+                                if (nf < T) push_near = bit_set(bmx, v) == 0;
                                 else push_far = bit_set(bmf, v) == 0;
                             }
                         }
@@ -236,17 +284,17 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
             ++r;
             __syncthreads();
         }
-        // ---------------- all keys below T are final
+        // ---------------- every key with f below T is final
         if (!full) {
             const uint64_t kd = key_load(keys + rdst);
-            if (kd != ORX_KEY_INF && kdist(kd) < T) break;
+            if (kd != ORX_KEY_INF && (double)kdist(kd) < T) break;  // h(dst) = 0
         }
         const uint32_t nf = ctl->fcnt;
         if (nf == 0) break;
-        // split pass 1: drop settled far entries, compact live ones, find min
-        if (tid == 0) { ctl->fcnt2 = 0; ctl->minbits = 0x7F800000u; ctl->phases++; }
+        // split pass 1: drop settled far entries, compact live ones, find min f
+        if (tid == 0) { ctl->fcnt2 = 0; ctl->minf = dbits((double)INFINITY); ctl->phases++; }
         __syncthreads();
-        float lmin = INFINITY;
+        double lmin = INFINITY;
         for (uint32_t base = 0; base < nf; base += BLOCK) {
             const uint32_t i = base + tid;
             const bool active = i < nf;
@@ -254,21 +302,23 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
             uint32_t v = 0;
             if (active) {
                 v = fA[i];
-                const float d = kdist(key_load(keys + v));
-                if (d < T) bit_clear(bmf, v);
-                else { keep = true; lmin = fminf(lmin, d); }
+                uint32_t lx, ly, z;
+                roi_decode(B, v, lx, ly, z);
+                const double f = (double)kdist(key_load(keys + v)) + box_h(H, B, lx, ly, z);
+                if (f < T) bit_clear(bmf, v);
+                else { keep = true; lmin = fmin(lmin, f); }
             }
             const int at = wave_append(keep, &ctl->fcnt2);
             if (keep) fB[at] = v;
         }
-        lmin = wave_min(lmin);
-        if (lane_id() == 0 && lmin < INFINITY) atomicMin(&ctl->minbits, __float_as_uint(lmin));
+        for (int o = 32; o > 0; o >>= 1) lmin = fmin(lmin, __shfl_xor(lmin, o));
+        if (lane_id() == 0 && lmin < (double)INFINITY) atomicMin(&ctl->minf, dbits(lmin));
         __syncthreads();
         const uint32_t nf2 = ctl->fcnt2;
         if (nf2 == 0) break;
-        const float mn = __uint_as_float(ctl->minbits);
-        float Tn = __fadd_rn(mn, delta);
-        if (!(Tn > mn)) Tn = nextafterf(mn, INFINITY);
+        const double mn = __longlong_as_double((long long)ctl->minf);
+        double Tn = mn + (double)delta;
+        if (!(Tn > mn)) Tn = nextafter(mn, (double)INFINITY);
         if (tid == 0) ctl->fcnt = 0;
         __syncthreads();
         // split pass 2: entries below the new threshold form round r's queue
@@ -281,8 +331,10 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
             uint32_t v = 0;
             if (active) {
                 v = fB[i];
-                const float d = kdist(key_load(keys + v));
-                if (d < Tn) { to_near = true; bit_clear(bmf, v); bit_set(bmr, v); }
+                uint32_t lx, ly, z;
+                roi_decode(B, v, lx, ly, z);
+                const double f = (double)kdist(key_load(keys + v)) + box_h(H, B, lx, ly, z);
+                if (f < Tn) { to_near = true; bit_clear(bmf, v); bit_set(bmr, v); }
                 else stay = true;
             }
             int at = wave_append(to_near, &ctl->qcnt[r % 3]);
@@ -297,21 +349,27 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
 }
 
 // Settled-set statistics of the finished search: the nodes a label-setting
-// search popping in (key, node id) order settles up to and including the
+// search popping in (f, key, node id) order settles up to and including the
 // sink (every reached node without a sink), and the in-ROI out-degree of
 // those popped before the sink.  ROI-local indices order like global ids.
+// Every node ordered before the sink has f <= f(dst) < T, so its key is final.
 template <int BLOCK>
-__device__ void box_stats(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rdst, bool full, Ctl* ctl) {
+__device__ void box_stats(const Lat& g, const Box& B, const Heur& H, const uint64_t* keys, uint32_t rdst, bool full,
+                          Ctl* ctl) {
     const uint64_t kd = full ? ORX_KEY_INF : key_load(keys + rdst);
+    const double fd = full ? (double)INFINITY : (double)kdist(kd);
     unsigned long long settled = 0, relaxed = 0;
     for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
         const uint64_t k = key_load(keys + r);
         if (k == ORX_KEY_INF) continue;
-        if (!full && (k > kd || (k == kd && r > rdst))) continue;
-        settled += 1;
-        if (!full && r == rdst) continue;
         uint32_t lx, ly, z;
         roi_decode(B, r, lx, ly, z);
+        if (!full) {
+            const double f = (double)kdist(k) + box_h(H, B, lx, ly, z);
+            if (f > fd || (f == fd && (k > kd || (k == kd && r > rdst)))) continue;
+        }
+        settled += 1;
+        if (!full && r == rdst) continue;
         const bool horiz = (g.hmask >> z) & 1u;
         for (int s = 0; s < 4; ++s) relaxed += roi_nbr(B, horiz, r, lx, ly, z, s) != UINT32_MAX;
     }
@@ -345,6 +403,18 @@ __device__ int32_t box_backtrace(const Lat& g, const Box& B, const uint64_t* key
         v = p;
     }
     return ORX_E_CYCLE;
+}
+
+__device__ __forceinline__ Heur make_heur(const Lat& g, const Params& P, int64_t dst, bool allowed) {
+    Heur H;
+    H.on = allowed && P.astar_cp > 0.0f && P.astar_cv > 0.0f && P.astar_gmax > 0.0f;
+    H.cp = P.astar_cp;
+    H.cv = P.astar_cv;
+    H.tz = (int32_t)(dst / g.XY);
+    const int64_t rem = dst - (int64_t)H.tz * g.XY;
+    H.ty = (int32_t)(rem / g.X);
+    H.tx = (int32_t)(rem - (int64_t)H.ty * g.X);
+    return H;
 }
 
 __device__ __forceinline__ Box make_box(const int32_t* b, int32_t L) {
@@ -393,21 +463,19 @@ __device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t
     return S;
 }
 
+// The per-net body of _route_all (unified_pathfinder.py:7949-8430), run by
+// one workgroup: rip-up, search, backtrace, commit.  Ends with a barrier.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, NetsDev nets, const int32_t* wave_nets,
-                                                           const int64_t* ws_off, orx_workspace ws, float delta,
-                                                           int32_t lds_nodes, int32_t flags,
-                                                           unsigned long long* stats) {
-    extern __shared__ uint64_t lds_keys[];
-    __shared__ Ctl ctl;
-    const int32_t k = wave_nets[blockIdx.x];
+__device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, int32_t k, const orx_workspace& ws,
+                          int64_t ws_off, float delta, int32_t lds_nodes, int32_t flags, unsigned long long* stats,
+                          uint64_t* lds_keys, Ctl& ctl) {
     const Box B = make_box(nets.box + 4 * (int64_t)k, g.L);
     const bool use_lds = (int64_t)B.R <= (int64_t)lds_nodes;
-    const SlotState S = slot_state(ws, ws_off[blockIdx.x], use_lds, lds_keys, lds_nodes);
+    const SlotState S = slot_state(ws, ws_off, use_lds, lds_keys, lds_nodes);
     int32_t* path = nets.path_arena + nets.path_off[k];
     const int32_t cap = nets.path_cap[k];
 
-    if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; }
+    if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; ctl.rounds = 0; ctl.phases = 0; ctl.items = 0; }
     // 1. rip up the previous path and re-price what it held
     const int32_t old_len = nets.path_len[k];
     if (old_len > 0) {
@@ -419,28 +487,41 @@ __global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, Nets
     // 2. search
     const uint32_t rsrc = global_to_roi(g, B, nets.src[k]);
     const uint32_t rdst = global_to_roi(g, B, nets.dst[k]);
+    int32_t len = 0;
     if (rsrc == UINT32_MAX || rdst == UINT32_MAX) {
         if (threadIdx.x == 0) { nets.status[k] = ORX_E_ARG; nets.path_len[k] = 0; }
-        return;
-    }
-    const bool full = (flags & ORX_F_FULL) != 0;
-    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB, &ctl);
-    if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, S.keys, rdst, full, &ctl);
-    // 3. backtrace
-    if (threadIdx.x == 0) {
-        int32_t len = box_backtrace(g, B, S.keys, rsrc, rdst, path, cap);
-        ctl.path_len = len;
-        nets.status[k] = len > 0 ? ORX_OK : len;
-        nets.path_len[k] = len > 0 ? len : 0;
-        nets.path_cost[k] = len > 0 ? kdist(key_load(S.keys + rdst)) : INFINITY;
-    }
-    __syncthreads();
-    const int32_t len = ctl.path_len;
-    // 4. commit and re-price
-    if (len > 0) {
-        apply_path_block<BLOCK>(g, path, len, +1);
+        len = ORX_E_ARG;
+    } else {
+        const bool full = (flags & ORX_F_FULL) != 0;
+        Heur H = make_heur(g, P, nets.dst[k], !full);
+        box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB,
+                        &ctl);
+        if (H.on) {
+            const uint64_t kd = key_load(S.keys + rdst);
+            if (kd != ORX_KEY_INF && !(kdist(kd) < P.astar_gmax)) {
+                // distances this large may round below the A* margin: redo plainly
+                H.on = false;
+                box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA,
+                                S.fB, &ctl);
+            }
+        }
+        if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
+        // 3. backtrace
+        if (threadIdx.x == 0) {
+            const int32_t l = box_backtrace(g, B, S.keys, rsrc, rdst, path, cap);
+            ctl.path_len = l;
+            nets.status[k] = l > 0 ? ORX_OK : l;
+            nets.path_len[k] = l > 0 ? l : 0;
+            nets.path_cost[k] = l > 0 ? kdist(key_load(S.keys + rdst)) : INFINITY;
+        }
         __syncthreads();
-        reprice_path_block<BLOCK>(g, P, path, len);
+        len = ctl.path_len;
+        // 4. commit and re-price
+        if (len > 0) {
+            apply_path_block<BLOCK>(g, path, len, +1);
+            __syncthreads();
+            reprice_path_block<BLOCK>(g, P, path, len);
+        }
     }
     if (stats && threadIdx.x == 0) {
         atomicAdd(stats + 0, ctl.settled);
@@ -451,6 +532,58 @@ __global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, Nets
         atomicAdd(stats + (len > 0 ? 5 : 6), 1ull);
         if (len > 0) atomicAdd(stats + 7, (unsigned long long)(len - 1));
     }
+    __syncthreads();
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, NetsDev nets, const int32_t* wave_nets,
+                                                           const int64_t* ws_off, orx_workspace ws, float delta,
+                                                           int32_t lds_nodes, int32_t flags,
+                                                           unsigned long long* stats) {
+    ORX_DYNAMIC_LDS(uint64_t, lds_keys);
+    ORX_SHARED(Ctl, ctl);
+    route_net<BLOCK>(g, P, nets, wave_nets[blockIdx.x], ws, ws_off[blockIdx.x], delta, lds_nodes, flags, stats,
+                     lds_keys, ctl);
+}
+
+// Persistent dependency-driven form: workgroups claim nets in schedule order
+// (a topological order of the dependency lists) from a global counter, wait
+// until every dependency of the claimed net is published for this epoch,
+// route it and publish it.  Every awaited net was claimed earlier by a
+// running workgroup, so the wait always ends.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void route_dag_kernel(Lat g, Params P, NetsDev nets, const int32_t* sched,
+                                                          int32_t n, const int32_t* dep_ptr, const int32_t* dep_ids,
+                                                          int32_t* done, int32_t epoch, uint32_t* next,
+                                                          const int64_t* ws_off, orx_workspace ws, float delta,
+                                                          int32_t lds_nodes, int32_t flags,
+                                                          unsigned long long* stats) {
+    ORX_DYNAMIC_LDS(uint64_t, lds_keys);
+    ORX_SHARED(Ctl, ctl);
+    ORX_SHARED(int32_t, s_pick);
+    for (;;) {
+        if (threadIdx.x == 0) s_pick = (int32_t)atomicAdd(next, 1u);
+        __syncthreads();
+        const int32_t i = s_pick;
+        if (i >= n) break;
+        const int32_t k = sched[i];
+        if (threadIdx.x == 0) {
+            for (int32_t d = dep_ptr[k]; d < dep_ptr[k + 1]; ++d) {
+                const int32_t p = dep_ids[d];
+                while (__hip_atomic_load(done + p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+                    __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        route_net<BLOCK>(g, P, nets, k, ws, ws_off[k], delta, lds_nodes, flags, stats, lds_keys, ctl);
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(done + k, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p) {
+    if (threadIdx.x == 0) p[0] = 0;
 }
 
 // Single box query writing ROI-local dist / parent arrays.
@@ -460,13 +593,13 @@ __global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws
                                                          int32_t lds_nodes, int32_t flags, float* dist,
                                                          int32_t* parent, unsigned long long* stats,
                                                          int32_t* status) {
-    extern __shared__ uint64_t lds_keys[];
-    __shared__ Ctl ctl;
+    ORX_DYNAMIC_LDS(uint64_t, lds_keys);
+    ORX_SHARED(Ctl, ctl);
     const int32_t bx[4] = {x0, y0, w, h};
     const Box B = make_box(bx, g.L);
     const bool use_lds = (int64_t)B.R <= (int64_t)lds_nodes;
     const SlotState S = slot_state(ws, 0, use_lds, lds_keys, lds_nodes);
-    if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; }
+    if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; ctl.rounds = 0; ctl.phases = 0; ctl.items = 0; }
     const uint32_t rsrc = global_to_roi(g, B, src);
     const bool full = (flags & ORX_F_FULL) != 0 || dst < 0;
     const uint32_t rdst = dst >= 0 ? global_to_roi(g, B, dst) : 0;
@@ -474,8 +607,10 @@ __global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws
         if (threadIdx.x == 0) *status = ORX_E_ARG;
         return;
     }
-    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB, &ctl);
-    if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, S.keys, rdst, full, &ctl);
+    Heur H;
+    H.on = false;
+    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB, &ctl);
+    if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
     for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
         const uint64_t k = key_load(S.keys + r);
         if (dist) dist[r] = k == ORX_KEY_INF ? INFINITY : kdist(k);
@@ -625,6 +760,7 @@ Params make_params(const orx_params* p) {
     P.node_hist_w = p->node_hist_w; P.edge_cap = p->edge_cap; P.node_cap = p->node_cap;
     P.hist_gain = p->hist_gain; P.hist_decay = p->hist_decay; P.hist_cap_mult = p->hist_cap_mult;
     P.node_hist_gain = p->node_hist_gain;
+    P.astar_cp = p->astar_cp; P.astar_cv = p->astar_cv; P.astar_gmax = p->astar_gmax;
     return P;
 }
 
@@ -717,6 +853,63 @@ int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* 
     default: return orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
     }
 #undef ORX_LAUNCH_ROUTE
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* nets, const int32_t* d_sched,
+                  int32_t n, const int32_t* d_dep_ptr, const int32_t* d_dep_ids, int32_t* d_done, int32_t epoch,
+                  uint32_t* d_counter, const int64_t* d_ws_off, int64_t max_nodes, const orx_workspace* ws,
+                  const orx_route_cfg* cfg, int32_t max_groups, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!p || !nets || !ws || !cfg || !d_sched || !d_dep_ptr || !d_done || !d_counter || !d_ws_off)
+        return orx_host::fail(ORX_E_ARG, "NULL argument");
+    // Switched off on the device: its first gfx950 run ended in an illegal
+    // address fault whose cause has not been found (the host-thread build of
+    // this kernel, with sequential and concurrent workgroups under
+    // AddressSanitizer, is clean and bit-exact).  Only that build, which
+    // defines ORX_HOST_THREAD_BUILD, may run it.
+#ifndef ORX_HOST_THREAD_BUILD
+    return orx_host::fail(ORX_E_ARG, "orx_route_dag is switched off on the device (unexplained fault)");
+#endif
+    if (n <= 0) return ORX_OK;
+    if (max_nodes >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI larger than 2^24 nodes");
+    if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
+    int32_t lds_nodes = cfg->lds_nodes < 0 ? 0 : cfg->lds_nodes;
+    if ((int64_t)lds_nodes > max_nodes) lds_nodes = (int32_t)((max_nodes + 63) / 64 * 64);
+    const size_t lds = lds_bytes_for(lds_nodes);
+    if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
+    NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
+               nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+    unsigned long long* st = (unsigned long long*)cfg->stats;
+    const hipStream_t s = (hipStream_t)stream;
+    const Lat g = make_lat(lat);
+    const Params P = make_params(p);
+    int dev = 0, cus = 0;
+    ORX_HIP_CHECK(hipGetDevice(&dev));
+    ORX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, d_counter);
+#define ORX_LAUNCH_DAG(BS)                                                                                \
+    do {                                                                                                  \
+        rc = set_lds(route_dag_kernel<BS>, lds);                                                          \
+        if (rc) return rc;                                                                                \
+        int per_cu = 0;                                                                                   \
+        ORX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, route_dag_kernel<BS>, BS, lds)); \
+        int grid = std::max(1, per_cu) * std::max(1, cus);                                                \
+        if (max_groups > 0) grid = std::min(grid, max_groups);                                            \
+        grid = std::min(grid, n);                                                                         \
+        hipLaunchKernelGGL(route_dag_kernel<BS>, dim3(grid), dim3(BS), lds, s, g, P, nd, d_sched, n,      \
+                           d_dep_ptr, d_dep_ids, d_done, epoch, d_counter, d_ws_off, *ws, cfg->delta,     \
+                           lds_nodes, cfg->flags, st);                                                    \
+    } while (0)
+    switch (cfg->block) {
+    case 256: ORX_LAUNCH_DAG(256); break;
+    case 512: ORX_LAUNCH_DAG(512); break;
+    case 1024: ORX_LAUNCH_DAG(1024); break;
+    default: return orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
+    }
+#undef ORX_LAUNCH_DAG
     ORX_HIP_CHECK(hipGetLastError());
     return ORX_OK;
 }

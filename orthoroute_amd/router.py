@@ -30,7 +30,8 @@ import torch
 from . import _lib
 from .lattice import LatticeSpec
 from .netgen import Netlist
-from .waves import color_waves, group_waves, net_distance, net_order, pres_schedule, resolve_pres_fac_max
+from .waves import (check_schedule, color_waves, group_waves, net_distance, net_order, plan_iteration,
+                    pres_schedule, resolve_pres_fac_max)
 
 
 @dataclass
@@ -53,6 +54,7 @@ class RouterConfig:
     delta: float = 1.0
     block: int = 256
     lds_nodes: int = 12288
+    astar: bool = True
     stop_at_zero_overuse: bool = False
     collect_stats: bool = True
 
@@ -60,14 +62,23 @@ class RouterConfig:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
         return pres_schedule(self.iterations, self.pres_init, self.pres_mult, pmax)
 
-    def params(self, pres: float) -> _lib.OrxParams:
+    def params(self, pres: float, astar=(0.0, 0.0, 0.0)) -> _lib.OrxParams:
         p = np.float32(pres)
         return _lib.OrxParams(
             pres_fac=float(p), base_w=self.base_w, hist_w=self.hist_w,
             node_pres=float(np.float32(self.node_pres_w) * p), node_hist_w=self.node_hist_w,
             edge_cap=self.edge_cap, node_cap=self.node_cap, hist_gain=self.hist_gain,
             hist_decay=self.hist_decay, hist_cap_mult=self.hist_cap_mult,
-            node_hist_gain=self.node_hist_gain)
+            node_hist_gain=self.node_hist_gain,
+            astar_cp=astar[0], astar_cv=astar[1], astar_gmax=astar[2])
+
+    def astar_bounds(self, base: np.ndarray) -> tuple[float, float, float]:
+        """(cp, cv, gmax) of the sink-directed heuristic, (0, 0, 0) when off."""
+        weights = (self.base_w, self.hist_w, self.node_pres_w, self.node_hist_w, self.hist_gain,
+                   self.node_hist_gain, self.hist_decay)
+        if not self.astar or min(weights) < 0 or self.pres_init < 0 or self.pres_mult < 0:
+            return (0.0, 0.0, 0.0)
+        return astar_bounds(base, self.base_w)
 
 
 @dataclass
@@ -97,6 +108,42 @@ class IterationStats:
     @property
     def overuse(self) -> int:
         return self.edge_over + self.node_over
+
+
+ASTAR_MARGIN_BITS = 8
+
+
+def _round_down_f32(x: float) -> np.float32:
+    f = np.float32(x)
+    if float(f) > x:
+        f = np.nextafter(f, np.float32(0))
+    return f
+
+
+def astar_bounds(base: np.ndarray, base_w: float, margin_bits: int = ASTAR_MARGIN_BITS):
+    """Per-move lower bounds of the entering weight for the A* heuristic.
+
+    Every entering weight is fl(fl(fl(b*base_w) + ...) + ...) + q with
+    non-negative terms, so it is at least fl(b*base_w).  The bounds used are
+    c' <= c_min * (1 - 2^-margin_bits); the margin m = c_min - c' covers half
+    an fp32 ulp of every distance below gmax = 2^(24 + floor(log2 m)), which
+    makes f = g + h monotone along tight arcs there."""
+    b = np.asarray(base, np.float32).reshape(-1, 4)
+    planar = b[:, 1:3][np.isfinite(b[:, 1:3])]
+    via = b[:, [0, 3]][np.isfinite(b[:, [0, 3]])]
+    if planar.size == 0:
+        return (0.0, 0.0, 0.0)
+    bw = np.float32(base_w)
+    cmin_p = float(np.float32(planar.min()) * bw)
+    cmin_v = float(np.float32(via.min()) * bw) if via.size else cmin_p
+    if cmin_p <= 0 or cmin_v <= 0:
+        return (0.0, 0.0, 0.0)
+    k = 1.0 - 2.0 ** -margin_bits
+    cp = _round_down_f32(cmin_p * k)
+    cv = _round_down_f32(cmin_v * k)
+    m = min(cmin_p - float(cp), cmin_v - float(cv))
+    gmax = 2.0 ** min(24 + int(np.floor(np.log2(m))), 120)
+    return (float(cp), float(cv), float(gmax))
 
 
 def _stream_handle(stream: torch.cuda.Stream | None) -> int:
@@ -165,6 +212,16 @@ class NetTables:
         self.path_len = torch.zeros(n, dtype=torch.int32, device=dev)
         self.path_cost = torch.full((n,), float("inf"), dtype=torch.float32, device=dev)
         self.status = torch.zeros(n, dtype=torch.int32, device=dev)
+        # per-net search workspace offsets (dependency-driven launch) and
+        # completion flags stamped with the iteration epoch
+        r64 = (R.astype(np.int64) + 63) // 64 * 64
+        ws_off = np.zeros(n, np.int64)
+        if n:
+            ws_off[1:] = np.cumsum(r64)[:-1]
+        self.ws_nodes = int(r64.sum())
+        self.ws_off = torch.from_numpy(ws_off).to(dev)
+        self.done = torch.full((max(1, n),), -1, dtype=torch.int32, device=dev)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
         self.struct = _lib.OrxNets(
             n, self.box.data_ptr(), self.src.data_ptr(), self.dst.data_ptr(), self.path_arena.data_ptr(),
             self.path_off.data_ptr(), self.path_cap.data_ptr(), self.path_len.data_ptr(),
@@ -173,7 +230,7 @@ class NetTables:
     def nbytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in
                    (self.box, self.src, self.dst, self.path_arena, self.path_off, self.path_cap,
-                    self.path_len, self.path_cost, self.status))
+                    self.path_len, self.path_cost, self.status, self.ws_off, self.done))
 
     def paths(self) -> list[np.ndarray]:
         arena = self.path_arena.cpu().numpy()
@@ -232,7 +289,9 @@ class DevicePathFinder:
         self.ws = Workspace(device)
         self.distance = net_distance(self.spec, netlist.src, netlist.dst)
         self.schedule = self.cfg.schedule(self.spec.L)
+        self.astar = self.cfg.astar_bounds(self.spec.base_costs() if base is None else base)
         self.iteration = 0
+        self.max_groups = 0
         self.history: list[IterationStats] = []
         self._route_stats = torch.zeros(8, dtype=torch.int64, device=self.device)
         self._iter_out = torch.zeros(8, dtype=torch.int64, device=self.device)
@@ -265,10 +324,11 @@ class DevicePathFinder:
         return net_order(iteration, self.distance, over, hops)
 
     def plan(self, order: np.ndarray, mode: str) -> list[np.ndarray]:
+        """Launch groups: one per net (plain) or one per disjoint-ROI wave."""
         if mode == "plain":
             return [np.asarray([k], np.int32) for k in order]
-        if mode != "batched":
-            raise ValueError("mode must be 'batched' or 'plain'")
+        if mode not in ("batched", "waves"):
+            raise ValueError("mode must be 'batched' ('waves') or 'plain'")
         wave = color_waves(self.netlist.box, order, self.spec.X, self.spec.Y)
         return group_waves(wave, order, self.nets.roi_nodes)
 
@@ -279,11 +339,13 @@ class DevicePathFinder:
             raise RuntimeError("schedule exhausted; raise cfg.iterations")
         it = self.iteration + 1
         pres = float(self.schedule[it - 1])
-        params = self.cfg.params(pres)
+        params = self.cfg.params(pres, self.astar)
         stream = self.stream if self.stream is not None else torch.cuda.current_stream()
         t0 = time.perf_counter()
         if order is None:
             order = self.order_for(it)
+        if mode == "dag":
+            raise RuntimeError("the dependency-driven launch (orx_route_dag) is switched off on the device")
         if waves is None:
             waves = self.plan(order, mode)
         # upload the wave plan once
@@ -331,6 +393,46 @@ class DevicePathFinder:
             settled=int(rs[0]), relaxed=int(rs[1]), items=int(rs[2]), rounds=int(rs[3]), phases=int(rs[4]),
             path_hops=int(rs[7]), waves=len(waves), seconds=dt, gpu_ms=ev0.elapsed_time(ev1),
             route_ms=evr0.elapsed_time(evr1))
+        self.iteration = it
+        self.history.append(st)
+        return st
+
+    def _run_dag(self, it, pres, params, order, stream, t0) -> IterationStats:
+        """One persistent dependency-driven launch for the whole iteration."""
+        nl = self.netlist
+        wave, sched, dep_ptr, dep_ids = plan_iteration(nl.box, order, self.spec.X, self.spec.Y, self.nets.roi_nodes)
+        check_schedule(sched, dep_ptr, dep_ids)
+        self.ws.ensure(self.nets.ws_nodes)
+        d_sched = torch.from_numpy(sched).to(self.device)
+        d_ptr = torch.from_numpy(dep_ptr).to(self.device)
+        d_ids = torch.from_numpy(dep_ids if len(dep_ids) else np.zeros(1, np.int32)).to(self.device)
+        self._route_stats.zero_()
+        ev0, ev1, evr0, evr1 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+        ev0.record(stream)
+        self.lat.reprice(params, stream)
+        rcfg = self.route_cfg()
+        s = self._s()
+        evr0.record(stream)
+        _lib.call("orx_route_dag", C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
+                  d_sched.data_ptr(), nl.n, d_ptr.data_ptr(), d_ids.data_ptr(), self.nets.done.data_ptr(), it,
+                  self.nets.counter.data_ptr(), self.nets.ws_off.data_ptr(), int(self.nets.roi_nodes.max()),
+                  C.byref(self.ws.struct), C.byref(rcfg), int(self.max_groups), s)
+        evr1.record(stream)
+        _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
+                  self._iter_out.data_ptr(), s)
+        ev1.record(stream)
+        out = self._iter_out.cpu().numpy()
+        rs = self._route_stats.cpu().numpy()
+        stream.synchronize()
+        dt = time.perf_counter() - t0
+        st = IterationStats(
+            iteration=it, pres_fac=pres, routed=int(rs[5]), failed=int(rs[6]),
+            edge_over=int(out[0]), edge_over_count=int(out[1]), node_over=int(out[2]),
+            node_over_count=int(out[3]), via_hops=int(out[4]), barrels=int(out[5]),
+            settled=int(rs[0]), relaxed=int(rs[1]), items=int(rs[2]), rounds=int(rs[3]), phases=int(rs[4]),
+            path_hops=int(rs[7]), waves=int(wave.max()) + 1 if len(wave) else 0, seconds=dt,
+            gpu_ms=ev0.elapsed_time(ev1), route_ms=evr0.elapsed_time(evr1),
+            extra={"launches": 1, "deps": int(len(dep_ids))})
         self.iteration = it
         self.history.append(st)
         return st

@@ -95,3 +95,44 @@ def check_disjoint(box: np.ndarray, nets: np.ndarray) -> bool:
     o = ox & oy
     np.fill_diagonal(o, False)
     return not bool(o.any())
+
+
+def plan_iteration(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes: np.ndarray):
+    """Native planner (orx_plan_iteration in liborx_hip.so): returns
+    (wave[n], sched[n], dep_ptr[n+1], dep_ids[]) for the dependency-driven
+    launch.  Runs on the host only."""
+    import ctypes as C
+
+    from . import _lib
+    lib = _lib.load()
+    n = len(order)
+    box = np.ascontiguousarray(box.reshape(-1), np.int32)
+    order = np.ascontiguousarray(order, np.int32)
+    rn = np.ascontiguousarray(roi_nodes, np.int64)
+    wave = np.empty(n, np.int32)
+    sched = np.empty(n, np.int32)
+    dep_ptr = np.empty(n + 1, np.int32)
+    cap = max(16, 8 * n)
+    while True:
+        dep_ids = np.empty(cap, np.int32)
+        total = lib.orx_plan_iteration(box.ctypes.data, order.ctypes.data, n, X, Y, rn.ctypes.data,
+                                       wave.ctypes.data, sched.ctypes.data, dep_ptr.ctypes.data,
+                                       dep_ids.ctypes.data, cap)
+        if total < 0:
+            raise ValueError(f"orx_plan_iteration rejected its input ({total})")
+        if total <= cap:
+            return wave, sched, dep_ptr, dep_ids[:total]
+        cap = int(total)
+
+
+def check_schedule(sched: np.ndarray, dep_ptr: np.ndarray, dep_ids: np.ndarray) -> None:
+    """Every dependency must come earlier in the schedule (the launch's
+    progress guarantee rests on it)."""
+    n = len(sched)
+    pos = np.full(n, -1, np.int64)
+    pos[sched] = np.arange(n)
+    if (pos < 0).any():
+        raise ValueError("schedule is not a permutation of the nets")
+    owner = np.repeat(np.arange(n), np.diff(dep_ptr))
+    if len(dep_ids) and not (pos[dep_ids] < pos[owner]).all():
+        raise ValueError("a dependency is scheduled after its dependant")

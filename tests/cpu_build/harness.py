@@ -1,0 +1,137 @@
+"""Runs PathFinder iterations of the kernel sources through the host-thread
+ASan build (route_driver) -- the same host logic as
+orthoroute_amd.router.DevicePathFinder (order, planner, per-net workspace),
+with every array on the host.  Test infrastructure."""
+from __future__ import annotations
+
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from orthoroute_amd.waves import check_schedule, net_distance, net_order, plan_iteration
+
+HERE = Path(__file__).resolve().parent
+DRIVER = HERE / "_out" / "route_driver"
+
+
+def build_driver() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return DRIVER
+
+
+class CpuBuildPathFinder:
+    def __init__(self, netlist, cfg, workdir: Path, mode: int = 0, max_groups: int = 0):
+        self.nl = netlist
+        self.cfg = cfg
+        self.spec = spec = netlist.spec
+        self.dir = Path(workdir)
+        self.dir.mkdir(parents=True, exist_ok=True)
+        self.mode = mode
+        self.max_groups = max_groups
+        N, n = spec.N, netlist.n
+        self.base = spec.base_costs()
+        self.hist = np.zeros(4 * N, np.float32)
+        self.occ = np.zeros(4 * N, np.int32)
+        self.wcost = np.zeros(4 * N, np.float32)
+        self.node_hist = np.zeros(N, np.float32)
+        self.node_use = np.zeros(N, np.int32)
+        self.via_col = np.zeros(spec.XY, np.int32)
+        R = netlist.roi_nodes().astype(np.int64)
+        self.R = R
+        self.path_cap = R.astype(np.int32)
+        self.path_off = np.zeros(n, np.int64)
+        self.path_off[1:] = np.cumsum(R)[:-1]
+        self.arena = np.zeros(int(R.sum()), np.int32)
+        self.path_len = np.zeros(n, np.int32)
+        self.path_cost = np.full(n, np.inf, np.float32)
+        self.status = np.zeros(n, np.int32)
+        r64 = (R + 63) // 64 * 64
+        self.ws_off = np.zeros(n, np.int64)
+        self.ws_off[1:] = np.cumsum(r64)[:-1]
+        self.ws_nodes = int(r64.sum())
+        self.done = np.full(n, -1, np.int32)
+        self.distance = net_distance(spec, netlist.src, netlist.dst)
+        self.schedule = cfg.schedule(spec.L)
+        self.astar = cfg.astar_bounds(self.base)
+        self.iteration = 0
+        self.nb = spec.neighbours()
+
+    def _net_overuse(self):
+        n = self.nl.n
+        over = np.zeros(n, np.int64)
+        hops = np.maximum(self.path_len - 1, 0).astype(np.int32)
+        for k in range(n):
+            p = self.arena[self.path_off[k]:self.path_off[k] + self.path_len[k]].astype(np.int64)
+            if len(p) < 2:
+                continue
+            s = np.argmax(self.nb[p[:-1]] == p[1:, None], axis=1)
+            o = self.occ[4 * p[:-1] + s] - self.cfg.edge_cap
+            over[k] = int(np.maximum(o, 0).sum())
+        return over, hops
+
+    def run_iteration(self, timeout: float = 600.0) -> dict:
+        it = self.iteration + 1
+        n = self.nl.n
+        if it > 1:
+            over, hops = self._net_overuse()
+        else:
+            over, hops = np.zeros(n, np.int64), np.zeros(n, np.int32)
+        order = net_order(it, self.distance, over, hops)
+        wave, sched, dep_ptr, dep_ids = plan_iteration(self.nl.box, order, self.spec.X, self.spec.Y, self.R)
+        check_schedule(sched, dep_ptr, dep_ids)
+        if self.mode == 1:
+            sched = order.astype(np.int32)
+        p = np.float32(self.schedule[it - 1])
+        c = self.cfg
+        meta = {
+            "X": self.spec.X, "Y": self.spec.Y, "L": self.spec.L, "hmask": self.spec.hmask, "n": n,
+            "arena": len(self.arena), "n_deps": len(dep_ids), "ws_nodes": self.ws_nodes,
+            "max_nodes": int(self.R.max()), "mode": self.mode, "epoch": it, "max_groups": self.max_groups,
+            "pres_fac": float(p), "base_w": c.base_w, "hist_w": c.hist_w,
+            "node_pres": float(np.float32(c.node_pres_w) * p), "node_hist_w": c.node_hist_w,
+            "edge_cap": c.edge_cap, "node_cap": c.node_cap, "hist_gain": c.hist_gain, "hist_decay": c.hist_decay,
+            "hist_cap_mult": c.hist_cap_mult, "node_hist_gain": c.node_hist_gain,
+            "astar_cp": self.astar[0], "astar_cv": self.astar[1], "astar_gmax": self.astar[2],
+            "delta": c.delta, "block": c.block, "lds_nodes": c.lds_nodes, "flags": 1,
+        }
+        with open(self.dir / "meta.txt", "w") as f:
+            for k, v in meta.items():
+                f.write(f"{k} {float(v)!r}\n")
+        arrays = {
+            "base": self.base, "hist": self.hist, "occ": self.occ, "wcost": self.wcost,
+            "node_hist": self.node_hist, "node_use": self.node_use, "via_col": self.via_col,
+            "box": np.ascontiguousarray(self.nl.box.reshape(-1), np.int32),
+            "src": self.nl.src.astype(np.int32), "dst": self.nl.dst.astype(np.int32),
+            "path_arena": self.arena, "path_off": self.path_off, "path_cap": self.path_cap,
+            "path_len": self.path_len, "path_cost": self.path_cost, "status": self.status,
+            "sched": sched.astype(np.int32), "dep_ptr": dep_ptr.astype(np.int32),
+            "dep_ids": dep_ids.astype(np.int32) if len(dep_ids) else np.zeros(1, np.int32),
+            "ws_off": self.ws_off, "done": self.done,
+        }
+        for k, a in arrays.items():
+            np.ascontiguousarray(a).tofile(self.dir / k)
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:halt_on_error=1")
+        res = subprocess.run([str(DRIVER), str(self.dir)], capture_output=True, text=True, timeout=timeout, env=env)
+        if res.returncode != 0:
+            raise RuntimeError(f"route_driver failed ({res.returncode}):\n{res.stderr[-6000:]}")
+
+        def rd(name, like):
+            return np.fromfile(self.dir / f"{name}.out", dtype=like.dtype, count=like.size)
+
+        self.hist, self.occ, self.wcost = rd("hist", self.hist), rd("occ", self.occ), rd("wcost", self.wcost)
+        self.node_hist, self.node_use = rd("node_hist", self.node_hist), rd("node_use", self.node_use)
+        self.via_col, self.arena = rd("via_col", self.via_col), rd("path_arena", self.arena)
+        self.path_len, self.path_cost = rd("path_len", self.path_len), rd("path_cost", self.path_cost)
+        self.status, self.done = rd("status", self.status), rd("done", self.done)
+        stats = np.fromfile(self.dir / "stats.out", np.int64)
+        out = np.fromfile(self.dir / "iter_out.out", np.int64)
+        self.iteration = it
+        return {"routed": int(stats[5]), "failed": int(stats[6]), "edge_over": int(out[0]),
+                "edge_over_count": int(out[1]), "node_over": int(out[2]), "node_over_count": int(out[3]),
+                "via_hops": int(out[4]), "barrels": int(out[5]), "settled": int(stats[0]), "relaxed": int(stats[1]),
+                "path_hops": int(stats[7])}
+
+    def paths(self):
+        return [self.arena[o:o + l].copy() for o, l in zip(self.path_off, self.path_len)]

@@ -61,11 +61,12 @@ def test_plain_64_ten_iterations_bit_exact(device, mode):
     assert dev.history[-1].overuse == orc.history[-1]["edge_over"] + orc.history[-1]["node_over"]
 
 
+@pytest.mark.parametrize("astar", [True, False])
 @pytest.mark.parametrize("lds_nodes", [0, 12288])
 @pytest.mark.parametrize("delta", [1.0, 3.0, float("inf")])
-def test_uneven_small_bit_exact(device, lds_nodes, delta):
+def test_uneven_small_bit_exact(device, lds_nodes, delta, astar):
     nl = uneven(96, 80, 8, 300, seed=11)
-    cfg = RouterConfig(iterations=4, delta=delta, lds_nodes=lds_nodes)
+    cfg = RouterConfig(iterations=4, delta=delta, lds_nodes=lds_nodes, astar=astar)
     _run_both(nl, cfg, 4, "batched", device)
 
 
@@ -97,3 +98,21 @@ def test_sssp_box_random_weights(device, lds_nodes):
         od, op, _, _, _ = oracle_sssp_box(spec, w, box, int(src))
         assert np.array_equal(dist.cpu().numpy().view(np.uint32), od.view(np.uint32))
         np.testing.assert_array_equal(parent.cpu().numpy(), op)
+
+
+@pytest.mark.skip(reason="orx_route_dag is switched off on the device: its first gfx950 run faulted "
+                         "(illegal address) and the cause is not found; covered by tests/test_cpu_build.py")
+@pytest.mark.parametrize("max_groups", [1, 3, 0])
+def test_dependency_launch_any_group_count(device, max_groups):
+    """The persistent launch gives the sequential result for any number of
+    resident workgroups (1 = fully serial claim order)."""
+    nl = uneven(80, 70, 6, 200, seed=13)
+    cfg = RouterConfig(iterations=3)
+    dev = DevicePathFinder(nl, cfg, device=device)
+    dev.max_groups = max_groups
+    orc = OraclePathFinder(nl, cfg)
+    for _ in range(3):
+        sd = dev.run_iteration("dag")
+        so = orc.run_iteration()
+        _assert_same_iteration(sd, so)
+        _assert_same_state(dev, orc)

@@ -1,0 +1,83 @@
+"""CPU checks of the oracle PathFinder driver: determinism, bookkeeping
+invariants, A*/Dijkstra equivalence and convergence on small boards."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OraclePathFinder
+from orthoroute_amd.netgen import plain_64, uneven
+from orthoroute_amd.router import RouterConfig
+
+
+def _recount(o):
+    """Occupancy, node use and via columns recomputed from the paths."""
+    spec = o.spec
+    occ = np.zeros_like(o.occ)
+    use = np.zeros_like(o.node_use)
+    col = np.zeros_like(o.via_col)
+    nb = spec.neighbours()
+    for p in o.paths():
+        use[p] += 1
+        prev_vert = False
+        for a, b in zip(p[:-1], p[1:]):
+            s = int(np.nonzero(nb[a] == b)[0][0])
+            occ[4 * a + s] += 1
+            occ[4 * b + 3 - s] += 1
+            vert = s in (0, 3)
+            if vert and not prev_vert:
+                col[a % spec.XY] += 1
+            prev_vert = vert
+    return occ, use, col
+
+
+def test_bookkeeping_matches_paths_and_paths_are_valid():
+    nl = uneven(60, 50, 6, 120, seed=8)
+    o = OraclePathFinder(nl, RouterConfig(iterations=3))
+    for _ in range(3):
+        o.run_iteration()
+    occ, use, col = _recount(o)
+    np.testing.assert_array_equal(occ, o.occ)
+    np.testing.assert_array_equal(use, o.node_use)
+    np.testing.assert_array_equal(col, o.via_col)
+    spec = nl.spec
+    for k, p in enumerate(o.paths()):
+        assert p[0] == nl.src[k] and p[-1] == nl.dst[k]
+        assert len(set(p.tolist())) == len(p)  # simple path
+        x, y, _ = spec.coords(p)
+        x0, y0, w, h = nl.box[k]
+        assert (x >= x0).all() and (x < x0 + w).all() and (y >= y0).all() and (y < y0 + h).all()
+
+
+def test_deterministic_across_runs():
+    nl = plain_64()
+    runs = []
+    for _ in range(2):
+        o = OraclePathFinder(nl, RouterConfig(iterations=3))
+        for _ in range(3):
+            o.run_iteration()
+        runs.append(o)
+    for a, b in zip(runs[0].paths(), runs[1].paths()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("make", [plain_64, lambda: uneven(120, 100, 8, 250, seed=2)])
+def test_astar_and_dijkstra_give_identical_results(make):
+    nl = make()
+    res = []
+    for astar in (False, True):
+        o = OraclePathFinder(nl, RouterConfig(iterations=4, astar=astar))
+        hist = [o.run_iteration() for _ in range(4)]
+        res.append((o, hist))
+    (a, ha), (b, hb) = res
+    for x, y in zip(a.paths(), b.paths()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.occ, b.occ)
+    assert np.array_equal(a.hist.view(np.uint32), b.hist.view(np.uint32))
+    assert np.array_equal(a.path_cost.view(np.uint32), b.path_cost.view(np.uint32))
+    assert [h["edge_over"] for h in ha] == [h["edge_over"] for h in hb]
+    assert sum(h["settled"] for h in hb) < sum(h["settled"] for h in ha)
+
+
+def test_overuse_falls_on_plain_64():
+    o = OraclePathFinder(plain_64(), RouterConfig(iterations=10))
+    over = [(d := o.run_iteration())["edge_over"] + d["node_over"] for _ in range(10)]
+    assert over[-1] < over[0] / 50
