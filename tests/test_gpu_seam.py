@@ -1,0 +1,112 @@
+"""GPU tests of the drop-in solver seam (orthoroute_amd.solver.HipDijkstra,
+generic CSR kernel) against the reference's own vectors and the oracle.
+
+Vectors restated from bbenchoff/OrthoRoute tests/test_gpu_fallback.py:
+104-117 (ROI-CSR prices the entered node: [7.0, 3.0]), 120-145 (seed pays
+its penalty: [1, 3]), 148-173 (fp32 absorption chain: [3, 2, 1, 0]),
+176-268 (random 6x6 grid, seed 20260727, cost equals float64 heapq,
+rel 1e-5).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.oracle import csr_path, sssp_csr
+from orthoroute_amd.lattice import LatticeSpec, in_rank_and_transpose
+from orthoroute_amd.solver import HipDijkstra, try_attach_hip
+from tests.test_reference_vectors import random_grid_cases, reference_cost, returned_cost
+
+pytestmark = pytest.mark.gpu
+
+
+def test_roi_prices_destination_node(device):
+    hip = HipDijkstra(indptr=np.array([0, 1, 2]), indices=np.array([1, 0]), device=device)
+    costs = np.array([2.0, 3.0], np.float32)
+    pen = np.array([0.0, 5.0], np.float32)
+    roi = np.array([0, 1], np.int32)
+    assert hip.find_path_roi_gpu(0, 1, costs, roi, roi, node_penalty=pen) == [0, 1]
+    # the ROI-local search pays 2 + 5 to enter node 1
+    res = hip.find_path_multisource_multisink([(0, 0.0)], [(1, 0.0)], costs, roi, roi, node_penalty=pen)
+    assert res[0] == [0, 1]
+
+
+def test_fullgraph_prices_source_seed_node(device):
+    hip = HipDijkstra(indptr=np.array([0, 1, 2, 2, 2]), indices=np.array([3, 3]), device=device)
+    path = hip.find_path_fullgraph_gpu_seeds(
+        costs=torch.tensor([1.0, 2.0], device=device), src_seeds=np.array([0, 1], np.int32),
+        dst_targets=np.array([3], np.int32), src_seed_costs=np.zeros(2, np.float32),
+        dst_target_costs=np.zeros(1, np.float32),
+        node_penalty=torch.tensor([100.0, 0.0, 0.0, 0.0], device=device))
+    assert path == [1, 3]
+
+
+def test_fullgraph_high_cost_rounding_does_not_cycle_parents(device):
+    hip = HipDijkstra(indptr=np.array([0, 1, 3, 5, 6]), indices=np.array([1, 0, 2, 1, 3, 2]), device=device)
+    path = hip.find_path_fullgraph_gpu_seeds(
+        costs=torch.full((6,), 0.4, device=device), src_seeds=np.array([3], np.int32),
+        dst_targets=np.array([0], np.int32), src_seed_costs=np.array([16_000_000.0], np.float32),
+        dst_target_costs=np.zeros(1, np.float32))
+    assert path == [3, 2, 1, 0]
+
+
+def test_fullgraph_matches_reference_dijkstra_on_random_costs(device):
+    indptr, indices, cases = random_grid_cases()
+    hip = HipDijkstra(indptr=indptr, indices=indices, device=device)
+    for costs, penalty, source, target in cases:
+        path = hip.find_path_fullgraph_gpu_seeds(
+            costs=torch.from_numpy(costs).to(device), src_seeds=np.array([source], np.int32),
+            dst_targets=np.array([target], np.int32), src_seed_costs=np.zeros(1, np.float32),
+            dst_target_costs=np.zeros(1, np.float32), node_penalty=torch.from_numpy(penalty).to(device))
+        assert path and path[0] == source and path[-1] == target
+        assert returned_cost(indptr, indices, path, costs, penalty) == pytest.approx(
+            reference_cost(indptr, indices, costs, penalty, source, target), rel=1e-5, abs=1e-5)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_roi_search_matches_oracle_on_lattice_csr(device, seed):
+    """Random costs and penalties on a lattice CSR with a random ROI: the
+    path equals the oracle's (canonical tie-break), bit for bit."""
+    spec = LatticeSpec(14, 12, 4)
+    rng = np.random.default_rng(seed)
+    indptr, indices, _, _ = spec.to_csr()
+    E = len(indices)
+    costs = rng.integers(2, 12, E).astype(np.float32) / np.float32(2.0)   # many equal-cost ties
+    roi_nodes = np.sort(rng.choice(spec.N, size=spec.N * 3 // 4, replace=False)).astype(np.int32)
+    g2r = np.full(spec.N, -1, np.int32)
+    g2r[roi_nodes] = np.arange(len(roi_nodes), dtype=np.int32)
+    pen = rng.integers(0, 4, len(roi_nodes)).astype(np.float32)
+    rank, in_indptr, in_src = in_rank_and_transpose(spec.N, indptr, indices)
+    hip = HipDijkstra(indptr=indptr, indices=indices, device=device)
+    for _ in range(5):
+        s, t = rng.choice(roi_nodes, 2, replace=False)
+        got = hip.find_path_roi_gpu(int(s), int(t), costs, roi_nodes, g2r, node_penalty=pen)
+        rc, best, dist, parent = sssp_csr(spec.N, indptr, indices, costs, rank, in_indptr, in_src,
+                                          roi_nodes=roi_nodes, g2r=g2r, pen=pen, seeds=[int(s)],
+                                          targets=[int(t)], seed_pen=False)
+        want = csr_path(best, parent, roi_nodes, g2r) if rc == 0 else None
+        assert got == (want if want and len(want) > 1 else None)
+
+
+def test_try_attach_hip_grafts_solver(device):
+    class Solver:  # a SimpleDijkstra stand-in with its two methods
+        def __init__(self):
+            self.cpu_calls = 0
+
+        def find_path_roi(self, *a, **k):
+            self.cpu_calls += 1
+            return None
+
+        def find_path_multisource_multisink(self, *a, **k):
+            self.cpu_calls += 1
+            return None
+
+    class Graph:
+        indptr = np.array([0, 1, 2, 2, 2])
+        indices = np.array([3, 3])
+
+    s = Solver()
+    assert try_attach_hip(s, Graph(), device=device)
+    roi = np.arange(4, dtype=np.int32)
+    path = s.find_path_roi(1, 3, np.array([1.0, 2.0], np.float32), roi, roi)
+    assert path == [1, 3] and s.cpu_calls == 0
+    assert s.gpu_solver is s.hip_solver
