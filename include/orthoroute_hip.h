@@ -118,6 +118,8 @@ typedef struct orx_route_cfg {
 
 #define ORX_F_STATS 1       /* count settled nodes / relaxed arcs per query  */
 #define ORX_F_FULL 2        /* no early exit: settle the whole ROI           */
+#define ORX_F_GRID 8        /* orx_sssp_box: grid-wide search (automatic     */
+                            /* above 2^18 ROI nodes)                         */
 #define ORX_F_SEED_PEN 4    /* CSR seeds pay their own node penalty (multi-  */
                             /* seed forms UP:2582-2583, CD:5589-5606)        */
 
@@ -199,7 +201,10 @@ int orx_apply_paths(const orx_lattice* lat, const orx_params* p, const int32_t* 
 /* One box-ROI query on the current wcost (the SSSP microbenchmark and the
  * array-reporting form of find_path_roi_gpu, CD:4951-5040).  Writes
  * ROI-local dist (fp32) and parent (global id, -1 root / unreached) arrays.
- * Small ROIs run in one workgroup; larger ones run grid-wide. */
+ * Small ROIs run in one workgroup; larger ones (or ORX_F_GRID) run grid-wide
+ * as a device-side state machine replayed from a captured hipGraph (the
+ * persistent kernel of PK:260-516 without inter-workgroup waits).
+ * Synchronous: returns after the arrays are written. */
 int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int32_t h,
                  int64_t src, int64_t dst, float* d_dist, int32_t* d_parent,
                  const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream);

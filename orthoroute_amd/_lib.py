@@ -22,6 +22,7 @@ ORX_E_NODEVICE = -5
 ORX_F_STATS = 1
 ORX_F_FULL = 2
 ORX_F_SEED_PEN = 4
+ORX_F_GRID = 8
 
 STATUS_NAMES = {
     ORX_OK: "ok",

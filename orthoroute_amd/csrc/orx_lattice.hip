@@ -22,6 +22,7 @@
 #include "orx_common.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include "../../include/orthoroute_hip.h"
 
 using namespace orx;
@@ -746,6 +747,271 @@ __global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, i
     out_hops[k] = len > 0 ? len - 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Grid-wide box search: one query over an ROI too large for one workgroup
+// (the 1000x1000x12 single-query microbenchmark, array-reporting form of
+// find_path_roi_gpu).  A device-side state machine advanced by repeated
+// launches of one step kernel; no workgroup ever waits on another:
+//   RELAX  : every workgroup relaxes a slice of the round's near queue;
+//   SPLIT1 : drop settled far entries, compact the live ones, min f;
+//   SPLIT2 : entries below the new threshold form the next near queue.
+// The last workgroup to finish a step (counted with an agent-scope atomic)
+// advances the state.  Launch sequences are replayed as a hipGraph by the
+// host.  Same keys, tie-break and near-far rule as box_sssp.
+// ---------------------------------------------------------------------------
+enum : uint32_t { GS_RELAX = 0, GS_SPLIT1 = 1, GS_SPLIT2 = 2, GS_DONE = 3 };
+
+struct GridCtl {
+    uint32_t state;
+    uint32_t r;
+    uint32_t qcnt[3];
+    uint32_t fcnt, fcnt2;
+    uint32_t blocks_done;
+    unsigned long long minf;
+    double T, Tn;
+    unsigned long long rounds, phases, items, settled, relaxed;
+};
+
+__device__ __forceinline__ uint64_t gkey_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t gkey_min(uint64_t* p, uint64_t v) {
+    return __hip_atomic_fetch_min((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gbit_set(uint32_t* bm, uint32_t i) {
+    const uint32_t b = 1u << (i & 31u);
+    return __hip_atomic_fetch_or(bm + (i >> 5), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & b;
+}
+__device__ __forceinline__ void gbit_clear(uint32_t* bm, uint32_t i) {
+    __hip_atomic_fetch_and(bm + (i >> 5), ~(1u << (i & 31u)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gload_u32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct GridArgs {
+    Lat g;
+    Box B;
+    uint32_t rsrc, rdst;
+    int32_t full;
+    float delta;
+    uint64_t* keys;
+    uint32_t *bmn0, *bmn1, *bmf, *qA, *qB, *fA, *fB;
+    GridCtl* ctl;
+};
+
+__global__ __launch_bounds__(256) void grid_init_kernel(GridArgs A) {
+    const uint32_t R = A.B.R, words = (R + 31) / 32;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x)
+        A.keys[i] = ORX_KEY_INF;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
+        A.bmn0[i] = 0; A.bmn1[i] = 0; A.bmf[i] = 0;
+    }
+}
+
+__global__ void grid_seed_kernel(GridArgs A) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    GridCtl* c = A.ctl;
+    c->state = GS_RELAX; c->r = 0;
+    c->qcnt[0] = 1; c->qcnt[1] = 0; c->qcnt[2] = 0;
+    c->fcnt = 0; c->fcnt2 = 0; c->blocks_done = 0;
+    c->minf = 0; c->T = (double)A.delta; c->Tn = 0.0;
+    if (!(c->T > 0.0)) c->T = nextafter(0.0, (double)INFINITY);
+    c->rounds = 0; c->phases = 0; c->items = 0; c->settled = 0; c->relaxed = 0;
+    A.keys[A.rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
+    A.qA[0] = A.rsrc;
+    A.bmn0[A.rsrc >> 5] |= 1u << (A.rsrc & 31u);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void grid_step_kernel(GridArgs A) {
+    GridCtl* ctl = A.ctl;
+    const uint32_t state = ctl->state;
+    if (state == GS_DONE) return;
+    const Box& B = A.B;
+    const Lat& g = A.g;
+    const uint32_t r = ctl->r;
+    const double T = ctl->T;
+    const uint32_t stride = gridDim.x * BLOCK;
+    const uint32_t first = blockIdx.x * BLOCK;
+    if (state == GS_RELAX) {
+        const uint32_t n = gload_u32(&ctl->qcnt[r % 3]);
+        uint32_t* cur = (r & 1) ? A.qB : A.qA;
+        uint32_t* nxt = (r & 1) ? A.qA : A.qB;
+        uint32_t* bmc = (r & 1) ? A.bmn1 : A.bmn0;
+        uint32_t* bmx = (r & 1) ? A.bmn0 : A.bmn1;
+        uint32_t* cnx = &ctl->qcnt[(r + 1) % 3];
+        for (uint32_t base = first; base < n; base += stride) {
+            const uint32_t i = base + threadIdx.x;
+            const bool active = i < n;
+            uint32_t u = 0, lx = 0, ly = 0, z = 0;
+            bool horiz = false;
+            uint64_t ku = ORX_KEY_INF;
+            float w4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+            uint32_t nb[4] = {UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
+            if (active) {
+                u = cur[i];
+                gbit_clear(bmc, u);
+                ku = gkey_load(A.keys + u);
+                roi_decode(B, u, lx, ly, z);
+                horiz = (g.hmask >> z) & 1u;
+                const float4 wv = reinterpret_cast<const float4*>(g.wcost)[roi_global(g, B, lx, ly, z)];
+                w4[0] = wv.x; w4[1] = wv.y; w4[2] = wv.z; w4[3] = wv.w;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) nb[s] = roi_nbr(B, horiz, u, lx, ly, z, s);
+            }
+            const float du = kdist(ku);
+            const uint32_t hu = khops(ku);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                bool push_near = false, push_far = false;
+                const uint32_t v = nb[s];
+                if (v != UINT32_MAX && !isinf(w4[s])) {
+                    const float nd = __fadd_rn(du, w4[s]);
+                    const uint64_t nk = kpack(nd, hu + 1, 3u - (uint32_t)s);
+                    if (nk < gkey_load(A.keys + v)) {
+                        const uint64_t old = gkey_min(A.keys + v, nk);
+                        if (nk < old) {
+                            if ((double)nd < T) push_near = gbit_set(bmx, v) == 0;
+                            else push_far = gbit_set(A.bmf, v) == 0;
+                        }
+                    }
+                }
+                int at = wave_append(push_near, cnx);
+                if (push_near) nxt[at] = v;
+                at = wave_append(push_far, &ctl->fcnt);
+                if (push_far) A.fA[at] = v;
+            }
+        }
+    } else if (state == GS_SPLIT1) {
+        const uint32_t nf = gload_u32(&ctl->fcnt);
+        double lmin = INFINITY;
+        for (uint32_t base = first; base < nf; base += stride) {
+            const uint32_t i = base + threadIdx.x;
+            bool keep = false;
+            uint32_t v = 0;
+            if (i < nf) {
+                v = A.fA[i];
+                const double f = (double)kdist(gkey_load(A.keys + v));
+                if (f < T) gbit_clear(A.bmf, v);
+                else { keep = true; lmin = fmin(lmin, f); }
+            }
+            const int at = wave_append(keep, &ctl->fcnt2);
+            if (keep) A.fB[at] = v;
+        }
+        for (int o = 32; o > 0; o >>= 1) lmin = fmin(lmin, __shfl_xor(lmin, o));
+        if (lane_id() == 0 && lmin < (double)INFINITY)
+            __hip_atomic_fetch_min(&ctl->minf, dbits(lmin), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {  // GS_SPLIT2
+        const uint32_t nf2 = gload_u32(&ctl->fcnt2);
+        const double Tn = ctl->Tn;
+        uint32_t* qr = (r & 1) ? A.qB : A.qA;
+        uint32_t* bmr = (r & 1) ? A.bmn1 : A.bmn0;
+        for (uint32_t base = first; base < nf2; base += stride) {
+            const uint32_t i = base + threadIdx.x;
+            bool to_near = false, stay = false;
+            uint32_t v = 0;
+            if (i < nf2) {
+                v = A.fB[i];
+                const double f = (double)kdist(gkey_load(A.keys + v));
+                if (f < Tn) { to_near = true; gbit_clear(A.bmf, v); gbit_set(bmr, v); }
+                else stay = true;
+            }
+            int at = wave_append(to_near, &ctl->qcnt[r % 3]);
+            if (to_near) qr[at] = v;
+            at = wave_append(stay, &ctl->fcnt);
+            if (stay) A.fA[at] = v;
+        }
+    }
+    // the last workgroup to finish advances the state machine
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t prev = atomicAdd(&ctl->blocks_done, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            ctl->blocks_done = 0;
+            if (state == GS_RELAX) {
+                ctl->rounds += 1;
+                ctl->items += gload_u32(&ctl->qcnt[r % 3]);
+                const uint32_t r1 = r + 1;
+                ctl->r = r1;
+                ctl->qcnt[(r1 + 1) % 3] = 0;
+                if (gload_u32(&ctl->qcnt[r1 % 3]) == 0) {
+                    bool done = false;
+                    if (!A.full) {
+                        const uint64_t kd = gkey_load(A.keys + A.rdst);
+                        done = kd != ORX_KEY_INF && (double)kdist(kd) < T;
+                    }
+                    if (done || gload_u32(&ctl->fcnt) == 0) {
+                        ctl->state = GS_DONE;
+                    } else {
+                        ctl->fcnt2 = 0;
+                        ctl->minf = dbits((double)INFINITY);
+                        ctl->phases += 1;
+                        ctl->state = GS_SPLIT1;
+                    }
+                }
+            } else if (state == GS_SPLIT1) {
+                if (gload_u32(&ctl->fcnt2) == 0) {
+                    ctl->state = GS_DONE;
+                } else {
+                    const double mn = __longlong_as_double((long long)__hip_atomic_load(
+                        &ctl->minf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                    double Tn = mn + (double)A.delta;
+                    if (!(Tn > mn)) Tn = nextafter(mn, (double)INFINITY);
+                    ctl->Tn = Tn;
+                    ctl->fcnt = 0;
+                    ctl->state = GS_SPLIT2;
+                }
+            } else {
+                ctl->T = ctl->Tn;
+                ctl->state = GS_RELAX;
+            }
+            __threadfence();
+        }
+    }
+}
+
+// dist / parent arrays and settled-set statistics of a finished grid search.
+__global__ __launch_bounds__(256) void grid_finish_kernel(GridArgs A, float* dist, int32_t* parent) {
+    const Box& B = A.B;
+    const Lat& g = A.g;
+    const uint64_t kd = A.full ? ORX_KEY_INF : A.keys[A.rdst];
+    unsigned long long settled = 0, relaxed = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < B.R; r += gridDim.x * blockDim.x) {
+        const uint64_t k = A.keys[r];
+        uint32_t lx, ly, z;
+        roi_decode(B, r, lx, ly, z);
+        const bool horiz = (g.hmask >> z) & 1u;
+        if (dist) dist[r] = k == ORX_KEY_INF ? INFINITY : kdist(k);
+        if (parent) {
+            int32_t p = -1;
+            if (k != ORX_KEY_INF && kslot(k) != ORX_ROOT_SLOT) {
+                const uint32_t pr = roi_nbr(B, horiz, r, lx, ly, z, (int)kslot(k));
+                if (pr != UINT32_MAX) {
+                    uint32_t plx, ply, pz;
+                    roi_decode(B, pr, plx, ply, pz);
+                    p = (int32_t)roi_global(g, B, plx, ply, pz);
+                }
+            }
+            parent[r] = p;
+        }
+        if (k == ORX_KEY_INF) continue;
+        if (!A.full && (k > kd || (k == kd && r > A.rdst))) continue;
+        settled += 1;
+        if (!A.full && r == A.rdst) continue;
+        for (int s = 0; s < 4; ++s) relaxed += roi_nbr(B, horiz, r, lx, ly, z, s) != UINT32_MAX;
+    }
+    settled = wave_sum(settled);
+    relaxed = wave_sum(relaxed);
+    if (lane_id() == 0) {
+        if (settled) atomicAdd(&A.ctl->settled, settled);
+        if (relaxed) atomicAdd(&A.ctl->relaxed, relaxed);
+    }
+}
+
 Lat make_lat(const orx_lattice* l) {
     Lat g;
     g.X = l->X; g.Y = l->Y; g.L = l->L; g.hmask = l->hmask;
@@ -788,6 +1054,109 @@ int set_lds(K kernel, size_t bytes) {
     ORX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return ORX_OK;
+}
+
+
+int32_t L_of(const orx_lattice* l) { return l->L; }
+
+__global__ void grid_stats_kernel(const GridCtl* c, unsigned long long* stats, int32_t* status, int32_t full,
+                                  const uint64_t* keys, uint32_t rdst) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (stats) {
+        stats[0] += c->settled; stats[1] += c->relaxed; stats[2] += c->items;
+        stats[3] += c->rounds; stats[4] += c->phases;
+    }
+    *status = (!full && keys[rdst] == ORX_KEY_INF) ? ORX_NO_PATH : ORX_OK;
+}
+
+// Host driver of the grid state machine: seed, replay a captured graph of
+// kGridSteps step launches until the device reports done, then write the
+// arrays.  Runs on a private stream after the caller's stream drained.
+constexpr int kGridSteps = 64;
+
+int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t L, uint32_t rsrc, uint32_t rdst,
+                  bool full, float* d_dist, int32_t* d_parent, const orx_workspace* ws, const orx_route_cfg* cfg,
+                  hipStream_t caller) {
+    GridArgs A;
+    A.g = g;
+    A.B.x0 = x0; A.B.y0 = y0; A.B.w = w; A.B.h = h; A.B.L = L;
+    A.B.wh = (uint32_t)w * (uint32_t)h; A.B.R = A.B.wh * (uint32_t)L;
+    A.B.dw = FastDiv((uint32_t)w); A.B.dwh = FastDiv(A.B.wh);
+    A.rsrc = rsrc; A.rdst = rdst; A.full = full ? 1 : 0; A.delta = cfg->delta;
+    const int64_t wn = ws->ws_nodes;
+    A.keys = ws->keys;
+    A.qA = ws->queues; A.qB = ws->queues + wn; A.fA = ws->queues + 2 * wn; A.fB = ws->queues + 3 * wn;
+    A.bmn0 = ws->bitmaps; A.bmn1 = ws->bitmaps + wn / 32; A.bmf = ws->bitmaps + 2 * (wn / 32);
+    int dev = 0, cus = 1;
+    ORX_HIP_CHECK(hipGetDevice(&dev));
+    ORX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ORX_HIP_CHECK(hipStreamSynchronize(caller));
+    hipStream_t ps = nullptr;
+    ORX_HIP_CHECK(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+    GridCtl* ctl = nullptr;
+    int32_t* d_status = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = ORX_OK;
+    hipError_t e = hipMallocAsync((void**)&ctl, sizeof(GridCtl), ps);
+    if (e == hipSuccess) e = hipMallocAsync((void**)&d_status, sizeof(int32_t), ps);
+    if (e == hipSuccess) {
+        A.ctl = ctl;
+        const int fill_grid = std::max(1, std::min(cus * 8, (int)((A.B.R + 255) / 256)));
+        hipLaunchKernelGGL(grid_init_kernel, dim3(fill_grid), dim3(256), 0, ps, A);
+        hipLaunchKernelGGL(grid_seed_kernel, dim3(1), dim3(64), 0, ps, A);
+        e = hipGetLastError();
+    }
+    const int block = cfg->block;
+    const int grid = std::max(1, cus * std::max(1, 512 / std::max(256, block)));
+    if (e == hipSuccess) e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int i = 0; i < kGridSteps; ++i) {
+            switch (block) {
+            case 512: hipLaunchKernelGGL(grid_step_kernel<512>, dim3(grid), dim3(512), 0, ps, A); break;
+            case 1024: hipLaunchKernelGGL(grid_step_kernel<1024>, dim3(grid), dim3(1024), 0, ps, A); break;
+            default: hipLaunchKernelGGL(grid_step_kernel<256>, dim3(grid), dim3(256), 0, ps, A); break;
+            }
+        }
+        hipError_t e2 = hipStreamEndCapture(ps, &graph);
+        e = e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) {
+        // bounded wait: a search that has not finished after the time limit
+        // reports an error instead of looping forever
+        const auto t0 = std::chrono::steady_clock::now();
+        uint32_t state = GS_RELAX;
+        for (;;) {
+            e = hipGraphLaunch(exec, ps);
+            if (e == hipSuccess) e = hipMemcpyAsync(&state, &ctl->state, sizeof(uint32_t), hipMemcpyDeviceToHost, ps);
+            if (e == hipSuccess) e = hipStreamSynchronize(ps);
+            if (e != hipSuccess || state == GS_DONE) break;
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300)) {
+                rc = orx_host::fail(ORX_E_OVERFLOW, "grid search did not finish within 300 s");
+                break;
+            }
+        }
+    }
+    if (e == hipSuccess && rc == ORX_OK) {
+        const int fin_grid = std::max(1, std::min(cus * 8, (int)((A.B.R + 255) / 256)));
+        hipLaunchKernelGGL(grid_finish_kernel, dim3(fin_grid), dim3(256), 0, ps, A, d_dist, d_parent);
+        hipLaunchKernelGGL(grid_stats_kernel, dim3(1), dim3(64), 0, ps, (const GridCtl*)ctl,
+                           (unsigned long long*)cfg->stats, d_status, A.full, (const uint64_t*)A.keys, rdst);
+        int32_t h_status = 0;
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&h_status, d_status, sizeof(int32_t), hipMemcpyDeviceToHost, ps);
+        if (e == hipSuccess) e = hipStreamSynchronize(ps);
+        if (e == hipSuccess) rc = h_status;
+    }
+    if (e != hipSuccess && rc == ORX_OK) rc = orx_host::check_hip(e, "orx_sssp_box (grid)");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ctl) (void)hipFreeAsync(ctl, ps);
+    if (d_status) (void)hipFreeAsync(d_status, ps);
+    (void)hipStreamSynchronize(ps);
+    (void)hipStreamDestroy(ps);
+    return rc;
 }
 
 }  // namespace
@@ -952,19 +1321,31 @@ int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int3
         return orx_host::fail(ORX_E_ARG, "box outside the lattice");
     const int64_t R = (int64_t)w * h * lat->L;
     if (R >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI larger than 2^24 nodes");
-    if (ws->ws_nodes < R) return orx_host::fail(ORX_E_ARG, "workspace smaller than the ROI");
+    if (ws->ws_nodes < R || ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "workspace smaller than the ROI");
+    const int64_t XY = (int64_t)lat->X * lat->Y;
+    auto to_roi = [&](int64_t gid) -> int64_t {
+        if (gid < 0 || gid >= XY * lat->L) return -1;
+        const int64_t z = gid / XY, rem = gid - z * XY, y = rem / lat->X, x = rem - y * lat->X;
+        if (x < x0 || x >= x0 + w || y < y0 || y >= y0 + h) return -1;
+        return (z * h + (y - y0)) * w + (x - x0);
+    };
+    if (to_roi(src) < 0 || (dst >= 0 && to_roi(dst) < 0)) return orx_host::fail(ORX_E_ARG, "src / dst outside the box");
+    const hipStream_t s = (hipStream_t)stream;
+    Lat g;
+    g.X = lat->X; g.Y = lat->Y; g.L = lat->L; g.hmask = lat->hmask; g.XY = XY;
+    g.base = lat->base; g.hist = lat->hist; g.occ = lat->occ; g.wcost = lat->wcost;
+    g.node_hist = lat->node_hist; g.node_use = lat->node_use; g.via_col = lat->via_col;
+    const bool use_grid = (cfg->flags & ORX_F_GRID) != 0 || R > (1 << 18);
+    if (use_grid) return sssp_box_grid(g, x0, y0, w, h, L_of(lat), (uint32_t)to_roi(src),
+                                       dst >= 0 ? (uint32_t)to_roi(dst) : 0u, dst < 0 || (cfg->flags & ORX_F_FULL),
+                                       d_dist, d_parent, ws, cfg, s);
     int32_t lds_nodes = cfg->lds_nodes < 0 ? 0 : cfg->lds_nodes;
     if ((int64_t)lds_nodes > R) lds_nodes = (int32_t)((R + 63) / 64 * 64);
     const size_t lds = lds_bytes_for(lds_nodes);
     if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
     int32_t* d_status = nullptr;
     int32_t h_status = 0;
-    ORX_HIP_CHECK(hipMallocAsync((void**)&d_status, sizeof(int32_t), (hipStream_t)stream));
-    const hipStream_t s = (hipStream_t)stream;
-    Lat g;
-    g.X = lat->X; g.Y = lat->Y; g.L = lat->L; g.hmask = lat->hmask; g.XY = (int64_t)lat->X * lat->Y;
-    g.base = lat->base; g.hist = lat->hist; g.occ = lat->occ; g.wcost = lat->wcost;
-    g.node_hist = lat->node_hist; g.node_use = lat->node_use; g.via_col = lat->via_col;
+    ORX_HIP_CHECK(hipMallocAsync((void**)&d_status, sizeof(int32_t), s));
     int rc = ORX_OK;
 #define ORX_LAUNCH_BOX(BS)                                                                                 \
     do {                                                                                                   \

@@ -135,3 +135,28 @@ class CpuBuildPathFinder:
 
     def paths(self):
         return [self.arena[o:o + l].copy() for o, l in zip(self.path_off, self.path_len)]
+
+
+SSSP_DRIVER = HERE / "_out" / "sssp_driver"
+
+
+def sssp_box_host_threads(spec, wcost: np.ndarray, box, src: int, dst: int, workdir: Path, grid: bool = True,
+                          delta: float = 2.0, block: int = 256, timeout: float = 600.0):
+    """One orx_sssp_box query through the host-thread build: (dist, parent, stats)."""
+    d = Path(workdir)
+    d.mkdir(parents=True, exist_ok=True)
+    x0, y0, w, h = (int(v) for v in box)
+    meta = {"X": spec.X, "Y": spec.Y, "L": spec.L, "hmask": spec.hmask, "x0": x0, "y0": y0, "w": w, "h": h,
+            "src": src, "dst": dst, "flags": (8 if grid else 0) | 1 | (2 if dst < 0 else 0), "block": block,
+            "delta": delta}
+    with open(d / "meta.txt", "w") as f:
+        for k, v in meta.items():
+            f.write(f"{k} {float(v)!r}\n")
+    np.ascontiguousarray(wcost, np.float32).tofile(d / "wcost")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
+    res = subprocess.run([str(SSSP_DRIVER), str(d)], capture_output=True, text=True, timeout=timeout, env=env)
+    if res.returncode != 0:
+        raise RuntimeError(f"sssp_driver failed ({res.returncode}):\n{res.stderr[-6000:]}")
+    R = w * h * spec.L
+    return (np.fromfile(d / "dist.out", np.float32, R), np.fromfile(d / "parent.out", np.int32, R),
+            np.fromfile(d / "stats.out", np.int64, 8))

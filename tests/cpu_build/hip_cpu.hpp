@@ -241,5 +241,47 @@ inline hipError_t hipMallocAsync(void** p, size_t b, hipStream_t) { *p = malloc(
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { free(p); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t b, int, hipStream_t) { memcpy(d, s, b); return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+// stream capture: launches on a capturing stream are recorded, and replayed
+// in order by hipGraphLaunch
+namespace hipcpu {
+inline bool g_capturing = false;
+inline std::vector<std::function<void()>>* g_capture = nullptr;
+template <typename K, typename... A>
+void launch_or_record(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
+    if (g_capturing) {
+        g_capture->push_back([=]() { launch(kernel, grid, block, lds, args...); });
+        return;
+    }
+    launch(kernel, grid, block, lds, args...);
+}
+}  // namespace hipcpu
+struct hipcpu_graph { std::vector<std::function<void()>> ops; };
+typedef hipcpu_graph* hipGraph_t;
+typedef hipcpu_graph* hipGraphExec_t;
+enum { hipStreamNonBlocking = 1, hipStreamCaptureModeThreadLocal = 1 };
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (hipStream_t)0x1; return 0; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) {
+    hipcpu::g_capturing = true;
+    hipcpu::g_capture = new std::vector<std::function<void()>>();
+    return 0;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    hipcpu::g_capturing = false;
+    *g = new hipcpu_graph{std::move(*hipcpu::g_capture)};
+    delete hipcpu::g_capture;
+    hipcpu::g_capture = nullptr;
+    return 0;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, void*, size_t) {
+    *e = new hipcpu_graph{g->ops};
+    return 0;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto& f : e->ops) f();
+    return 0;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return 0; }
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return 0; }
 #define hipLaunchKernelGGL(kernel, grid, block, lds, stream, ...) \
-    hipcpu::launch(kernel, dim3(grid), dim3(block), (size_t)(lds), __VA_ARGS__)
+    hipcpu::launch_or_record(kernel, dim3(grid), dim3(block), (size_t)(lds), __VA_ARGS__)

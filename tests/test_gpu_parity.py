@@ -116,3 +116,20 @@ def test_dependency_launch_any_group_count(device, max_groups):
         so = orc.run_iteration()
         _assert_same_iteration(sd, so)
         _assert_same_state(dev, orc)
+
+
+@pytest.mark.parametrize("delta", [1.0, 4.0, float("inf")])
+def test_sssp_grid_random_weights(device, delta):
+    """Grid-wide search (ORX_F_GRID): dist and parent arrays bit-exact."""
+    from orthoroute_amd import _lib
+    spec = LatticeSpec(60, 50, 6)
+    w = spec.random_edge_costs(3)
+    lat = LatticeState(spec, device=device)
+    lat.wcost.copy_(torch.from_numpy(w))
+    box = (3, 2, 50, 45)
+    stats = torch.zeros(8, dtype=torch.int64, device=device)
+    for src in (spec.node(10, 10, 0), spec.node(40, 30, 5)):
+        dist, parent, rc = sssp_box(lat, box, int(src), full=True, delta=delta, grid=True, stats=stats)
+        od, op, _, ns, nr = oracle_sssp_box(spec, w, box, int(src))
+        assert np.array_equal(dist.cpu().numpy().view(np.uint32), od.view(np.uint32))
+        np.testing.assert_array_equal(parent.cpu().numpy(), op)
