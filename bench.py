@@ -1,17 +1,30 @@
 """Benchmark of the PathFinder hot path on one MI355X.
 
-A step is one PathFinder iteration (rip-up, route and commit every net, then
-the overuse / history update) over the 800x800x20 lattice with 8000
-lognormal-length two-pin nets (seed 3), batched as disjoint-ROI waves.
-W warm-up iterations run first, then K timed iterations; the timed region
+A step is one PathFinder iteration over the 800x800x20 lattice with 8000
+lognormal-length two-pin nets (seed 3), batched as disjoint-ROI waves:
+- rip-up, route and commit every net;
+- then the overuse / history update.
+
+W warm-up iterations run first, then K timed iterations. The timed region
 starts with every array already resident in HBM.
 
-Printed (one JSON line): nets routed/s of the batched form (``value``),
-GTEPS, the plain one-net-per-launch form's iteration time on the same GPU
-and the speed-up, the roofline of the route kernel (algorithmic bytes of the
-label-setting search: 12 B per relaxed arc + 16 B per settled node, over the
-kernel time measured with HIP events on its stream), and a CPU baseline (the
-oracle, one core, on a bounded sample of the same workload).
+Printed: one JSON line.
+- ``value``: nets routed/s of the batched form.
+- GTEPS.
+- The plain one-net-per-launch form on the same GPU (iteration 1, bit-exact
+  check of paths) and the speed-up.
+- ``roofline`` of the route kernel: algorithmic bytes of the label-setting
+  search (12 B per relaxed arc + 16 B per settled node) over the kernel time,
+  measured with HIP events on the stream the waves run on.
+- ``cpu_baseline``: the oracle on one host core, on iteration 1 of the same
+  workload.
+- ``sssp_microbench``: 64 full queries on the 1000x1000x12 lattice, one query
+  per launch sequence.
+- ``serving``: a Poisson stream of routing jobs through the
+  continuous-batching server, with throughput and p50/p99 job latency.
+
+``--copy-probe`` only runs a plain device copy of known size. It is the
+calibration run for rocprofv3 FETCH_SIZE / WRITE_SIZE counters.
 """
 from __future__ import annotations
 
@@ -27,6 +40,11 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _cpu_model() -> str:
@@ -39,21 +57,136 @@ def _cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(nl, cfg) -> dict:
-    """Oracle (C restatement of the reference CPU path), 1 core, on iteration 1
-    of the same board restricted to every third net."""
+def cpu_baseline(nl, cfg, stride: int) -> dict:
+    """Oracle (C restatement of the reference CPU path), 1 core, iteration 1
+    on every ``stride``-th net of the workload."""
     from oracle.oracle import OraclePathFinder
-    sub_idx = list(range(0, nl.n, 3))
-    from orthoroute_amd.netgen import Netlist
-    sub = Netlist(nl.spec, nl.src[sub_idx], nl.dst[sub_idx], nl.box[sub_idx], nl.name + "[::3]")
+    sub = nl.select(list(range(0, nl.n, stride)), f"::{stride}") if stride > 1 else nl
     o = OraclePathFinder(sub, cfg)
     t0 = time.perf_counter()
     d = o.run_iteration()
     dt = time.perf_counter() - t0
+    what = "all nets" if stride == 1 else f"every {stride}th net"
     return {"value": d["routed"] / dt, "unit": "nets/s", "cores": 1, "kind": "port",
-            "sample": f"iteration 1 of {sub.name} ({sub.n} nets, every third net of the workload), "
-                      f"{dt:.1f} s on {_cpu_model()} (nproc {os.cpu_count()})",
-            "gteps": d["relaxed"] / dt / 1e9}
+            "sample": f"PathFinder iteration 1 of {nl.name} on {sub.n} nets ({what}), {dt:.1f} s "
+                      f"on {_cpu_model()} (nproc {os.cpu_count()})",
+            "gteps": d["relaxed"] / dt / 1e9, "seconds": dt}
+
+
+def copy_probe(mib: int = 256, reps: int = 5) -> None:
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "reps": reps}))
+
+
+def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1) -> dict:
+    """Full queries on 1000x1000x12 (costs U[1,4) per physical edge, a seeded
+    5% x10), endpoints uniform on layer 0, one query per launch sequence."""
+    import numpy as np
+    import torch
+    from orthoroute_amd.lattice import LatticeSpec
+    from orthoroute_amd.router import LatticeState, RouterConfig, Workspace
+    from orthoroute_amd.sssp import sssp_box
+
+    spec = LatticeSpec(1000, 1000, 12)
+    base = spec.random_edge_costs(seed)
+    lat = LatticeState(spec, base=base)
+    lat.reprice(RouterConfig().params(0.5))
+    ws = Workspace()
+    ws.ensure(spec.N)
+    rng = np.random.default_rng(seed + 100)
+    pairs = rng.integers(0, spec.XY, size=(n_queries, 2))
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+    box = (0, 0, spec.X, spec.Y)
+    # warm-up query (not timed)
+    sssp_box(lat, box, int(pairs[0, 0]), int(pairs[0, 1]), full=True, delta=delta, block=block, ws=ws,
+             stats=stats, grid=True)
+    stats.zero_()
+    torch.cuda.synchronize()
+    times = []
+    check = None
+    for q in range(n_queries):
+        t = time.perf_counter()
+        dist, parent, rc = sssp_box(lat, box, int(pairs[q, 0]), int(pairs[q, 1]), full=True, delta=delta,
+                                    block=block, ws=ws, stats=stats, grid=True)
+        times.append(time.perf_counter() - t)
+        if q == 0:
+            check = _check_path_cost(lat, dist, parent, int(pairs[q, 0]), int(pairs[q, 1]))
+    st = stats.cpu().numpy()
+    total = float(sum(times))
+    relaxed, settled = int(st[1]), int(st[0])
+    alg = 12 * relaxed + 16 * settled
+    return {
+        "workload": "sssp_1000x1000x12_full_query", "queries": n_queries, "delta": delta, "block": block,
+        "ms_per_query": total / n_queries * 1e3, "p50_ms": float(np.percentile(times, 50)) * 1e3,
+        "gteps": relaxed / total / 1e9, "relaxed_per_query": relaxed / n_queries,
+        "settled_per_query": settled / n_queries, "rounds_per_query": int(st[3]) / n_queries,
+        "path_cost_check": check,
+        "roofline": {"bound": "hbm", "achieved": alg / total / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / total / 1e9 / HBM_PEAK_GBS, "kernel": "grid_step_kernel",
+                     "algorithmic_bytes_per_query": alg / n_queries},
+    }
+
+
+def _check_path_cost(lat, dist, parent, src: int, dst: int) -> bool:
+    """The sink's fp32 distance equals the fp32 sum of entering weights along
+    its parent chain (a size-independent property of the result)."""
+    import numpy as np
+    spec = lat.spec
+    d = dist.cpu().numpy()
+    p = parent.cpu().numpy()
+    w = lat.wcost.cpu().numpy().reshape(-1, 4)
+    nb = spec.neighbours()
+    path = [dst]
+    while path[-1] != src:
+        u = int(p[path[-1]])
+        if u < 0 or len(path) > spec.N:
+            return False
+        path.append(u)
+    path.reverse()
+    acc = np.float32(0.0)
+    for u, v in zip(path[:-1], path[1:]):
+        s = int(np.flatnonzero(nb[u] == v)[0])
+        acc = np.float32(acc + w[u, s])
+    return bool(acc == d[dst])
+
+
+def serving_bench(n_jobs: int, rate: float, seed: int, max_iterations: int, tuned: dict,
+                  time_limit: float) -> dict:
+    from orthoroute_amd.router import RouterConfig
+    from orthoroute_amd.serve import RoutingServer, poisson_jobs
+
+    t0 = time.perf_counter()
+    jobs = poisson_jobs(n_jobs, rate, seed=seed, max_iterations=max_iterations)
+    gen_s = time.perf_counter() - t0
+    cfg = RouterConfig(iterations=max_iterations, stop_at_zero_overuse=True, delta=tuned["delta"],
+                       block=tuned["block"], lds_nodes=tuned["lds_nodes"])
+    srv = RoutingServer(cfg, max_slots=8)
+    summ = srv.run(jobs, time_limit=time_limit)
+    summ.update({
+        "arrival_rate_jobs_per_s": rate, "seed": seed, "max_iterations": max_iterations,
+        "job_generation_s": gen_s,
+        "jobs_detail": [{"board": j.netlist.name, "arrival_s": round(j.arrival, 3),
+                         "latency_s": round(j.latency, 3), "iterations": j.iterations,
+                         "final_overuse": j.overuse_history[-1] if j.overuse_history else None}
+                        for j in sorted(srv.finished, key=lambda j: j.job_id)],
+    })
+    return summ
+
+
+def _traffic():
+    """HBM bytes per route_wave_kernel dispatch from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+        return float(t["hbm_bytes_per_dispatch"]), t
+    except (OSError, ValueError, KeyError):
+        return None, None
 
 
 def main() -> int:
@@ -64,65 +197,119 @@ def main() -> int:
     ap.add_argument("--workload", default="uneven_800")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--no-sssp", action="store_true")
+    ap.add_argument("--sssp-queries", type=int, default=64)
+    ap.add_argument("--sssp-delta", type=float, default=16.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=6)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-iterations", type=int, default=50)
+    ap.add_argument("--serve-seed", type=int, default=4)
+    ap.add_argument("--graph", action="store_true", help="capture each iteration's wave launches in a hipGraph")
+    ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--delta", type=float, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--lds-nodes", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the path is single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs the path on a "
+              f"single GPU (multi-GPU sharding is a design, not an implementation)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        copy_probe()
+        return 0
 
     import torch
     from orthoroute_amd import _lib, netgen
-    from orthoroute_amd.router import DevicePathFinder, RouterConfig
     from orthoroute_amd.autotune import load_tuned
+    from orthoroute_amd.router import DevicePathFinder, RouterConfig
 
     _lib.init_device()
     nl = getattr(netgen, args.workload)()
     tuned = load_tuned(nl)
-    cfg = RouterConfig(iterations=args.warmup + args.steps,
-                       delta=args.delta if args.delta is not None else tuned["delta"],
-                       block=args.block if args.block is not None else tuned["block"],
-                       lds_nodes=args.lds_nodes if args.lds_nodes is not None else tuned["lds_nodes"])
+    delta = args.delta if args.delta is not None else tuned["delta"]
+    block = args.block if args.block is not None else tuned["block"]
+    lds_nodes = args.lds_nodes if args.lds_nodes is not None else tuned["lds_nodes"]
+    cfg = RouterConfig(iterations=args.warmup + args.steps, delta=delta, block=block, lds_nodes=lds_nodes, graph=args.graph)
 
+    _log(f"{nl.name}: {args.warmup} warm-up + {args.steps} timed iterations (delta {delta}, block {block})")
     dev = DevicePathFinder(nl, cfg)
     first = None
+    first_paths = None
     for _ in range(args.warmup):
         st = dev.run_iteration("batched")
-        first = first or st
+        if first is None:
+            first, first_paths = st, dev.paths()
+    # The timed iterations run without the settled / relaxed counting pass;
+    # the counts come from a replay of the same iterations with it on (every
+    # configuration computes the same result, so the counts are those of the
+    # timed work).
+    snap = dev.snapshot()
+    dev.cfg.collect_stats = False
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     timed = [dev.run_iteration("batched") for _ in range(args.steps)]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    first = first or timed[0]
+    digest_timed = dev.digest()
+    dev.restore(snap)
+    dev.cfg.collect_stats = True
+    counted = [dev.run_iteration("batched") for _ in range(args.steps)]
+    if dev.digest() != digest_timed:
+        raise RuntimeError("the counting replay did not reproduce the timed iterations")
+    if first is None:
+        first = timed[0]
     wall = t1 - t0
     routed = sum(s.routed for s in timed)
-    relaxed = sum(s.relaxed for s in timed)
-    settled = sum(s.settled for s in timed)
+    relaxed = sum(s.relaxed for s in counted)
+    settled = sum(s.settled for s in counted)
     route_s = sum(s.route_ms for s in timed) / 1e3
     waves = sum(s.waves for s in timed)
     alg_bytes = 12 * relaxed + 16 * settled
     achieved = alg_bytes / route_s / 1e9
     footprint = dev.footprint_bytes()
+    _log(f"batched: {wall / args.steps * 1e3:.1f} ms/iteration, overuse {timed[-1].overuse}")
     del dev
     torch.cuda.empty_cache()
 
     plain = None
     if not args.no_plain:
-        pdev = DevicePathFinder(nl, RouterConfig(iterations=1, delta=cfg.delta, block=cfg.block,
-                                                 lds_nodes=cfg.lds_nodes))
+        _log("plain one-net-per-launch iteration 1")
+        pdev = DevicePathFinder(nl, RouterConfig(iterations=1, delta=delta, block=block, lds_nodes=lds_nodes))
         pst = pdev.run_iteration("plain")
+        same = None
+        if first_paths is not None:
+            pp = pdev.paths()
+            same = bool(len(pp) == len(first_paths) and all((a.shape == b.shape) and (a == b).all()
+                                                            for a, b in zip(pp, first_paths)))
         plain = {"iteration_ms": pst.seconds * 1e3, "launches": pst.waves,
-                 "batched_iteration_ms": first.seconds * 1e3,
-                 "speedup": pst.seconds / first.seconds,
-                 "bit_exact_iteration1": bool(pst.overuse == first.overuse and pst.routed == first.routed)}
+                 "batched_iteration_ms": first.seconds * 1e3, "speedup": pst.seconds / first.seconds,
+                 "route_kernel_ms_plain": pst.route_ms, "route_kernel_ms_batched": first.route_ms,
+                 "bit_exact_iteration1": bool(same and pst.overuse == first.overuse and pst.routed == first.routed)}
         del pdev
         torch.cuda.empty_cache()
 
-    cpu = None if args.no_cpu else cpu_baseline(nl, cfg)
+    sssp = None
+    if not args.no_sssp:
+        _log(f"SSSP microbench: {args.sssp_queries} queries on 1000x1000x12")
+        sssp = sssp_microbench(args.sssp_queries, args.sssp_delta, 256)
+        torch.cuda.empty_cache()
 
+    serving = None
+    if not args.no_serve:
+        _log(f"serving: {args.serve_jobs} jobs at {args.serve_rate} jobs/s")
+        serving = serving_bench(args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_iterations, tuned,
+                                time_limit=300.0)
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        _log("CPU baseline (oracle, 1 core)")
+        cpu = cpu_baseline(nl, cfg, args.cpu_stride)
+
+    traffic, traffic_src = _traffic()
+    n_dispatch = waves
     out = {
         "metric": "nets routed/s (batched disjoint-ROI PathFinder iteration)",
         "value": routed / wall,
@@ -138,19 +325,24 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": nl.name, "lattice": [nl.spec.X, nl.spec.Y, nl.spec.L], "nets": nl.n,
                    "roi_margin": 4, "iterations_timed": [timed[0].iteration, timed[-1].iteration],
-                   "delta": cfg.delta, "block": cfg.block, "lds_nodes": cfg.lds_nodes,
+                   "delta": delta, "block": block, "lds_nodes": lds_nodes, "graph": args.graph,
                    "parallelism": "1 GPU, disjoint-ROI waves"},
         "gteps": relaxed / route_s / 1e9,
         "waves_per_iteration": waves / args.steps,
         "overuse_last": timed[-1].overuse,
         "hbm_footprint_gb": footprint / 1e9,
         "plain_vs_batched": plain,
-        "p99_job_latency_s": None,
+        "p99_job_latency_s": serving["p99_latency_s"] if serving else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "route_wave_kernel", "algorithmic_bytes_per_iteration": alg_bytes / args.steps,
-                     "kernel_ms_per_iteration": route_s * 1e3 / args.steps},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "route_wave_kernel",
+                     "algorithmic_bytes_per_launch": alg_bytes / n_dispatch,
+                     "kernel_ms_per_launch": route_s * 1e3 / n_dispatch,
+                     "launches_per_iteration": n_dispatch / args.steps,
+                     "traffic_source": str(TRAFFIC_FILE.relative_to(REPO)) if traffic_src else None},
         "cpu_baseline": cpu,
+        "sssp_microbench": sssp,
+        "serving": serving,
     }
     print(json.dumps(out))
     return 0

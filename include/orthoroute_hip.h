@@ -158,6 +158,29 @@ int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* 
                    int64_t max_nodes, const orx_workspace* ws, const orx_route_cfg* cfg,
                    orx_stream_t stream);
 
+/* The whole routing step of one iteration as a launch sequence: wave i is
+ * the nets d_nets[h_wave_ptr[i] .. h_wave_ptr[i+1]) with workspace offsets
+ * d_ws_off[same range], routed as by orx_route_wave, waves in order on one
+ * stream (the per-net loop of _route_all, UP:7949-8430, batched).
+ * h_wave_ptr[n_waves+1] and h_wave_max[n_waves] (largest ROI of each wave)
+ * are HOST arrays. */
+int orx_route_waves(const orx_lattice* lat, const orx_params* p, const orx_nets* nets,
+                    const int32_t* d_nets, const int64_t* d_ws_off, int32_t n_waves,
+                    const int32_t* h_wave_ptr, const int64_t* h_wave_max,
+                    const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream);
+
+/* Launch-sequence capture into a hipGraph (no reference counterpart: the
+ * reference launches each CuPy kernel from Python, CD:2968-3043).
+ * orx_graph_begin returns a private capture stream; orx_* calls enqueued on
+ * it are recorded, not run; orx_graph_end instantiates them;
+ * orx_graph_launch replays the graph on a stream; orx_graph_destroy frees it
+ * once no launch of it is pending. */
+typedef void* orx_graph_t;
+int orx_graph_begin(orx_stream_t* capture_stream);
+int orx_graph_end(orx_stream_t capture_stream, orx_graph_t* graph);
+int orx_graph_launch(orx_graph_t graph, orx_stream_t stream);
+int orx_graph_destroy(orx_graph_t graph);
+
 /* Route every net of one iteration in ONE persistent launch.  d_sched[n]
  * is a topological order of the nets; net k may start once every net in
  * d_dep_ids[d_dep_ptr[k] .. d_dep_ptr[k+1]) has published d_done[id] == epoch

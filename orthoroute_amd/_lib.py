@@ -90,6 +90,12 @@ SIGNATURES = [
     ("orx_lattice_end_iteration", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), P, P]),
     ("orx_route_wave", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
                                  C.c_int32, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
+    ("orx_route_waves", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
+                                  C.c_int32, P, P, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
+    ("orx_graph_begin", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("orx_graph_end", C.c_int, [P, C.POINTER(C.c_void_p)]),
+    ("orx_graph_launch", C.c_int, [P, P]),
+    ("orx_graph_destroy", C.c_int, [P]),
     ("orx_route_dag", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, C.c_int32, P, P,
                                 P, C.c_int32, P, P, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg),
                                 C.c_int32, P]),

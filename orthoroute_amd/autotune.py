@@ -1,24 +1,23 @@
 """Launch-parameter autotuner: bucket width delta, workgroup size and the
 ROI size up to which search keys live in LDS.
 
-``tune`` times candidate configurations on a sample of one iteration's waves
-(the first ``sample_waves`` waves, which hold the largest ROIs, plus an even
-stride over the rest) and keeps the fastest.  Results persist in
+``tune`` times every candidate configuration on the same full PathFinder
+iteration of the workload and keeps the fastest.  Results persist in
 ``tuned.json`` beside this file, keyed by device architecture and workload
 shape.  Results never depend on these parameters (every configuration
-computes the same canonical keys); only time does.
+computes the same canonical keys); only time does, and the tuner checks it.
+
+    python -m orthoroute_amd.autotune uneven_800
 """
 from __future__ import annotations
 
-import ctypes as C
 import itertools
 import json
+import sys
 from pathlib import Path
 
-import numpy as np
-
 TUNED_PATH = Path(__file__).resolve().parent / "tuned.json"
-DEFAULTS = {"delta": 4.0, "block": 1024, "lds_nodes": 12288}
+DEFAULTS = {"delta": 16.0, "block": 1024, "lds_nodes": 16384}
 
 
 def _key(nl, arch: str = "gfx950") -> str:
@@ -31,7 +30,7 @@ def load_tuned(nl) -> dict:
         table = json.loads(TUNED_PATH.read_text())
     except (OSError, ValueError):
         return dict(DEFAULTS)
-    return dict(table.get(_key(nl), DEFAULTS))
+    return {**DEFAULTS, **table.get(_key(nl), {})}
 
 
 def save_tuned(nl, params: dict) -> None:
@@ -43,48 +42,49 @@ def save_tuned(nl, params: dict) -> None:
     TUNED_PATH.write_text(json.dumps(table, indent=1, sort_keys=True))
 
 
-def tune(nl, deltas=(1.0, 2.0, 4.0, 8.0, float("inf")), blocks=(256, 512, 1024),
-         lds_nodes=(0, 8192, 12288), sample_waves: int = 12, stride: int = 8, save: bool = True,
-         log=print) -> dict:
-    """Time every configuration on the same sample of waves of iteration 1
-    (each candidate starts from the same fresh state)."""
+def tune(nl, deltas=(4.0, 8.0, 16.0, 32.0), blocks=(256, 512, 1024), lds_nodes=(8192, 16384),
+         iteration: int = 2, save: bool = True, log=print) -> dict:
+    """Time every configuration on the same full iteration (``iteration``,
+    default 2: the first congested one) of ``nl``.
+
+    One job is routed up to the iteration before, its device state is
+    snapshotted, and each candidate restores the snapshot and routes the
+    iteration.  Every configuration must produce the same result (checked by
+    digest); only the route-kernel time, from HIP events, differs."""
+    import dataclasses
+
     import torch
 
-    from . import _lib
-    from .router import DevicePathFinder, RouterConfig, _wave_offsets
+    from .router import DevicePathFinder, RouterConfig
 
+    base_cfg = RouterConfig(iterations=iteration, collect_stats=False, **DEFAULTS)
+    job = DevicePathFinder(nl, base_cfg)
+    for _ in range(iteration - 1):
+        job.run_iteration()
+    snap = job.snapshot()
     results = []
+    ref_digest = None
     for delta, block, lds in itertools.product(deltas, blocks, lds_nodes):
-        cfg = RouterConfig(iterations=1, delta=delta, block=block, lds_nodes=lds, collect_stats=False)
-        d = DevicePathFinder(nl, cfg)
-        order = d.order_for(1)
-        waves = d.plan(order, "batched")
-        pick = list(range(min(sample_waves, len(waves)))) + list(range(sample_waves, len(waves), stride))
-        params = cfg.params(float(d.schedule[0]), d.astar)
-        d.lat.reprice(params)
-        rcfg = d.route_cfg()
-        s = torch.cuda.current_stream()
-        total = 0.0
-        for wi in pick:
-            w = waves[wi]
-            off, need = _wave_offsets(d.nets.roi_nodes, w)
-            d.ws.ensure(need)
-            dn = torch.from_numpy(w.astype(np.int32)).to(d.device)
-            do = torch.from_numpy(off).to(d.device)
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(s)
-            _lib.call("orx_route_wave", C.byref(d.lat.struct), C.byref(params), C.byref(d.nets.struct),
-                      dn.data_ptr(), do.data_ptr(), len(w), int(d.nets.roi_nodes[w].max()), C.byref(d.ws.struct),
-                      C.byref(rcfg), s.cuda_stream)
-            e1.record(s)
-            e1.synchronize()
-            total += e0.elapsed_time(e1)
-        results.append((total, {"delta": delta, "block": block, "lds_nodes": lds}))
-        log(f"[autotune] delta={delta} block={block} lds_nodes={lds}: {total:.2f} ms on {len(pick)} waves")
-        del d
-        torch.cuda.empty_cache()
+        job.restore(snap)
+        job.cfg = dataclasses.replace(base_cfg, delta=delta, block=block, lds_nodes=lds)
+        st = job.run_iteration()
+        dg = job.digest()
+        if ref_digest is None:
+            ref_digest = dg
+        elif dg != ref_digest:
+            raise RuntimeError(f"delta={delta} block={block} lds_nodes={lds} changed the result")
+        results.append((st.route_ms, {"delta": delta, "block": block, "lds_nodes": lds}))
+        log(f"[autotune] delta={delta} block={block} lds_nodes={lds}: route {st.route_ms:.2f} ms "
+            f"({st.waves} launches)")
+    del job
+    torch.cuda.empty_cache()
     best = min(results, key=lambda r: r[0])[1]
     if save:
         save_tuned(nl, best)
     return best
+
+
+if __name__ == "__main__":
+    from . import netgen
+    name = sys.argv[1] if len(sys.argv) > 1 else "uneven_800"
+    print(json.dumps(tune(getattr(netgen, name)(), log=lambda m: print(m, flush=True))))

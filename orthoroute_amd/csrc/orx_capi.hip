@@ -40,6 +40,48 @@ int orx_init(void) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return orx_host::fail(ORX_E_NODEVICE, std::string("device is ") + prop.gcnArchName +
                                                    ", this library is built for gfx950 only");
+    return orx_host::prepare_lattice_kernels();
+}
+
+// Launch-sequence capture: a private non-blocking stream records every orx_*
+// call enqueued on it between orx_graph_begin and orx_graph_end into one
+// executable graph.  Nothing recorded runs until orx_graph_launch.
+int orx_graph_begin(orx_stream_t* capture_stream) {
+    if (!capture_stream) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    hipStream_t ps = nullptr;
+    ORX_HIP_CHECK(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(ps);
+        return orx_host::check_hip(e, "hipStreamBeginCapture");
+    }
+    *capture_stream = ps;
+    return ORX_OK;
+}
+
+int orx_graph_end(orx_stream_t capture_stream, orx_graph_t* graph) {
+    if (!capture_stream || !graph) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    hipStream_t ps = (hipStream_t)capture_stream;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamEndCapture(ps, &g);
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(ps);
+    if (e != hipSuccess) return orx_host::check_hip(e, "orx_graph_end");
+    *graph = exec;
+    return ORX_OK;
+}
+
+int orx_graph_launch(orx_graph_t graph, orx_stream_t stream) {
+    if (!graph) return orx_host::fail(ORX_E_ARG, "NULL graph");
+    ORX_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph, (hipStream_t)stream));
+    return ORX_OK;
+}
+
+int orx_graph_destroy(orx_graph_t graph) {
+    if (!graph) return ORX_OK;
+    ORX_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph));
     return ORX_OK;
 }
 

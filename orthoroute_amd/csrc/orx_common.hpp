@@ -78,6 +78,24 @@ __device__ __forceinline__ int wave_append(bool want, uint32_t* counter) {
     return want ? (int)(base + lanes_below(m)) : -1;
 }
 
+// Wavefront-aggregated append of a variable number of entries (0..7 per
+// lane): the exclusive prefix of the counts comes from three ballot
+// bit-slices and mbcnt, and the wave takes its range with one LDS atomic.
+// Uniform control flow required.  Returns the lane's first slot (only
+// meaningful when cnt > 0).
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t* counter) {
+    const uint64_t m0 = __ballot(cnt & 1u), m1 = __ballot(cnt & 2u), m2 = __ballot(cnt & 4u);
+    const uint32_t total = (uint32_t)__popcll(m0) + 2u * (uint32_t)__popcll(m1) + 4u * (uint32_t)__popcll(m2);
+    if (total == 0) return 0;
+    const uint32_t before = lanes_below(m0) + 2u * lanes_below(m1) + 4u * lanes_below(m2);
+    const uint64_t any = m0 | m1 | m2;
+    const uint32_t leader = (uint32_t)__ffsll((long long)any) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    return base + before;
+}
+
 // Cost formulas: must match the oracle bit for bit (contract(off) above, and
 // the library is built with -ffp-contract=off).
 struct Params {
@@ -148,6 +166,7 @@ namespace orx_host {
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
+int prepare_lattice_kernels();  // raise the dynamic-LDS limit of the search kernels once
 }  // namespace orx_host
 
 #define ORX_HIP_CHECK(expr)                                              \

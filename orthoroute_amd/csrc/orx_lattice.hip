@@ -22,6 +22,7 @@
 #include "orx_common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include "../../include/orthoroute_hip.h"
 
@@ -197,10 +198,21 @@ __device__ __forceinline__ unsigned long long dbits(double d) { return (unsigned
 // Near-far delta-stepping over one ROI box, one workgroup, bucketed by
 // f = g + h (h = 0 without a heuristic).
 // ---------------------------------------------------------------------------
+// Entries of the LDS ring that holds the first part of a round's near queue.
+constexpr uint32_t kRing = 2048;
+
+__device__ __forceinline__ uint32_t ring_get(const uint32_t* lq, const uint32_t* gq, uint32_t i) {
+    return i < kRing ? lq[i] : gq[i];
+}
+__device__ __forceinline__ void ring_put(uint32_t* lq, uint32_t* gq, uint32_t i, uint32_t v) {
+    if (i < kRing) lq[i] = v;
+    else gq[i] = v;
+}
+
 template <int BLOCK>
 __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rdst, bool full, float delta,
                          const Heur& H, uint64_t* keys, uint32_t* bmn0, uint32_t* bmn1, uint32_t* bmf, uint32_t* qA,
-                         uint32_t* qB, uint32_t* fA, uint32_t* fB, Ctl* ctl) {
+                         uint32_t* qB, uint32_t* lqA, uint32_t* lqB, uint32_t* fA, uint32_t* fB, Ctl* ctl) {
     const uint32_t tid = threadIdx.x;
     const uint32_t R = B.R;
     const uint32_t nwords = (R + 31) / 32;
@@ -221,7 +233,7 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
     }
     if (tid == 0) {
         keys[rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
-        qA[0] = rsrc;
+        lqA[0] = rsrc;
         bit_set(bmn0, rsrc);
     }
     __syncthreads();
@@ -232,8 +244,10 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
         for (;;) {
             const uint32_t n = ctl->qcnt[r % 3];
             if (n == 0) break;
-            uint32_t* cur = (r & 1) ? qB : qA;
+            const uint32_t* cur = (r & 1) ? qB : qA;
             uint32_t* nxt = (r & 1) ? qA : qB;
+            const uint32_t* lcur = (r & 1) ? lqB : lqA;
+            uint32_t* lnxt = (r & 1) ? lqA : lqB;
             uint32_t* bmc = (r & 1) ? bmn1 : bmn0;
             uint32_t* bmx = (r & 1) ? bmn0 : bmn1;
             uint32_t* cnx = &ctl->qcnt[(r + 1) % 3];
@@ -241,13 +255,17 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
             for (uint32_t base = 0; base < n; base += BLOCK) {
                 const uint32_t i = base + tid;
                 const bool active = i < n;
+                // The memory operations of a lane's four arcs are issued as
+                // four independent batches (loads, then key atomics, then
+                // bitmap atomics) so each step waits for memory once, not
+                // once per arc.
                 uint32_t u = 0, lx = 0, ly = 0, z = 0;
                 bool horiz = false;
                 uint64_t ku = ORX_KEY_INF;
                 float w4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
                 uint32_t nb[4] = {UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
                 if (active) {
-                    u = cur[i];
+                    u = ring_get(lcur, cur, i);
                     bit_clear(bmc, u);
                     ku = key_load(keys + u);
                     roi_decode(B, u, lx, ly, z);
@@ -260,26 +278,45 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
                 }
                 const float du = kdist(ku);
                 const uint32_t hu = khops(ku);
+                float nd[4];
+                uint64_t nk[4], old[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    bool push_near = false, push_far = false;
-                    const uint32_t v = nb[s];
-                    if (v != UINT32_MAX && !isinf(w4[s])) {
-                        const float nd = __fadd_rn(du, w4[s]);
-                        const uint64_t nk = kpack(nd, hu + 1, 3u - (uint32_t)s);
-                        if (nk < key_load(keys + v)) {
-                            const uint64_t old = key_min(keys + v, nk);
-                            if (nk < old) {
-                                const double nf = (double)nd + nbr_h(H, B, horiz, lx, ly, z, s);
-                                if (nf < T) push_near = bit_set(bmx, v) == 0;
-                                else push_far = bit_set(bmf, v) == 0;
-                            }
-                        }
+                    nd[s] = __fadd_rn(du, w4[s]);
+                    nk[s] = kpack(nd[s], hu + 1, 3u - (uint32_t)s);
+                    old[s] = 0;
+                    if (nb[s] != UINT32_MAX && !isinf(w4[s])) old[s] = key_min(keys + nb[s], nk[s]);
+                }
+                bool to_near[4], to_far[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    to_near[s] = false;
+                    to_far[s] = false;
+                    if (nk[s] < old[s]) {
+                        const double nf = (double)nd[s] + nbr_h(H, B, horiz, lx, ly, z, s);
+                        if (nf < T) to_near[s] = true;
+                        else to_far[s] = true;
                     }
-                    int at = wave_append(push_near, cnx);
-                    if (push_near) nxt[at] = v;
-                    at = wave_append(push_far, &ctl->fcnt);
-                    if (push_far) fA[at] = v;
+                }
+                uint32_t was[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    was[s] = 1;
+                    if (to_near[s]) was[s] = bit_set(bmx, nb[s]);
+                    else if (to_far[s]) was[s] = bit_set(bmf, nb[s]);
+                }
+                uint32_t cn = 0, cf = 0;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    cn += (to_near[s] && was[s] == 0) ? 1u : 0u;
+                    cf += (to_far[s] && was[s] == 0) ? 1u : 0u;
+                }
+                uint32_t an = wave_append_n(cn, cnx);
+                uint32_t af = wave_append_n(cf, &ctl->fcnt);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (to_near[s] && was[s] == 0) ring_put(lnxt, nxt, an++, nb[s]);
+                    if (to_far[s] && was[s] == 0) fA[af++] = nb[s];
                 }
             }
             ++r;
@@ -324,6 +361,7 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
         __syncthreads();
         // split pass 2: entries below the new threshold form round r's queue
         uint32_t* qr = (r & 1) ? qB : qA;
+        uint32_t* lqr = (r & 1) ? lqB : lqA;
         uint32_t* bmr = (r & 1) ? bmn1 : bmn0;
         for (uint32_t base = 0; base < nf2; base += BLOCK) {
             const uint32_t i = base + tid;
@@ -339,7 +377,7 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
                 else stay = true;
             }
             int at = wave_append(to_near, &ctl->qcnt[r % 3]);
-            if (to_near) qr[at] = v;
+            if (to_near) ring_put(lqr, qr, (uint32_t)at, v);
             at = wave_append(stay, &ctl->fcnt);
             if (stay) fA[at] = v;
         }
@@ -437,29 +475,46 @@ __device__ __forceinline__ uint32_t global_to_roi(const Lat& g, const Box& B, in
     return ((uint32_t)z * (uint32_t)B.h + (uint32_t)(y - B.y0)) * (uint32_t)B.w + (uint32_t)(x - B.x0);
 }
 
-// Search-state pointers of one wave slot.
+// Search-state pointers of one wave slot.  The dynamic LDS of a search
+// workgroup is laid out as
+//   [ near ring A: kRing u32 ][ near ring B: kRing u32 ][ region ]
+// The near queue of a round lives in the ring while it has at most kRing
+// entries and spills to the global queue beyond that.  The region holds the
+// keys and the three bitmaps when the ROI has at most lds_nodes nodes and
+// they fit, else only the bitmaps when those fit; the rest stays in the
+// global workspace.
 struct SlotState {
     uint64_t* keys;
     uint32_t *bmn0, *bmn1, *bmf, *qA, *qB, *fA, *fB;
+    uint32_t *lqA, *lqB;
 };
 
-__device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t off, bool lds, uint64_t* lds_keys,
-                                                int32_t lds_nodes) {
+__device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t off, uint32_t R, uint32_t* lds,
+                                                int32_t region_bytes, int32_t lds_nodes) {
     SlotState S;
     S.qA = ws.queues + off;
     S.qB = ws.queues + ws.ws_nodes + off;
     S.fA = ws.queues + 2 * ws.ws_nodes + off;
     S.fB = ws.queues + 3 * ws.ws_nodes + off;
-    if (lds) {
-        S.keys = lds_keys;
-        uint32_t* bm = reinterpret_cast<uint32_t*>(lds_keys + lds_nodes);
-        const int32_t words = (lds_nodes + 31) / 32;
+    S.lqA = lds;
+    S.lqB = lds + kRing;
+    uint32_t* region = lds + 2 * kRing;
+    const uint32_t words = (R + 31) / 32;
+    const int64_t bm_bytes = 3ll * words * 4;
+    const int64_t key_bytes = 8ll * R;
+    if ((int64_t)R <= (int64_t)lds_nodes && key_bytes + bm_bytes <= (int64_t)region_bytes) {
+        S.keys = reinterpret_cast<uint64_t*>(region);
+        uint32_t* bm = reinterpret_cast<uint32_t*>(S.keys + R);
         S.bmn0 = bm; S.bmn1 = bm + words; S.bmf = bm + 2 * words;
     } else {
         S.keys = ws.keys + off;
-        S.bmn0 = ws.bitmaps + off / 32;
-        S.bmn1 = ws.bitmaps + ws.ws_nodes / 32 + off / 32;
-        S.bmf = ws.bitmaps + 2 * (ws.ws_nodes / 32) + off / 32;
+        if (bm_bytes <= (int64_t)region_bytes) {
+            S.bmn0 = region; S.bmn1 = region + words; S.bmf = region + 2 * words;
+        } else {
+            S.bmn0 = ws.bitmaps + off / 32;
+            S.bmn1 = ws.bitmaps + ws.ws_nodes / 32 + off / 32;
+            S.bmf = ws.bitmaps + 2 * (ws.ws_nodes / 32) + off / 32;
+        }
     }
     return S;
 }
@@ -468,11 +523,10 @@ __device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t
 // one workgroup: rip-up, search, backtrace, commit.  Ends with a barrier.
 template <int BLOCK>
 __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, int32_t k, const orx_workspace& ws,
-                          int64_t ws_off, float delta, int32_t lds_nodes, int32_t flags, unsigned long long* stats,
-                          uint64_t* lds_keys, Ctl& ctl) {
+                          int64_t ws_off, float delta, int32_t lds_nodes, int32_t lds_region, int32_t flags,
+                          unsigned long long* stats, uint32_t* lds, Ctl& ctl) {
     const Box B = make_box(nets.box + 4 * (int64_t)k, g.L);
-    const bool use_lds = (int64_t)B.R <= (int64_t)lds_nodes;
-    const SlotState S = slot_state(ws, ws_off, use_lds, lds_keys, lds_nodes);
+    const SlotState S = slot_state(ws, ws_off, B.R, lds, lds_region, lds_nodes);
     int32_t* path = nets.path_arena + nets.path_off[k];
     const int32_t cap = nets.path_cap[k];
 
@@ -495,14 +549,14 @@ __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, in
     } else {
         const bool full = (flags & ORX_F_FULL) != 0;
         Heur H = make_heur(g, P, nets.dst[k], !full);
-        box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB,
+        box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA, S.fB,
                         &ctl);
         if (H.on) {
             const uint64_t kd = key_load(S.keys + rdst);
             if (kd != ORX_KEY_INF && !(kdist(kd) < P.astar_gmax)) {
                 // distances this large may round below the A* margin: redo plainly
                 H.on = false;
-                box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA,
+                box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA,
                                 S.fB, &ctl);
             }
         }
@@ -539,12 +593,12 @@ __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, in
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, NetsDev nets, const int32_t* wave_nets,
                                                            const int64_t* ws_off, orx_workspace ws, float delta,
-                                                           int32_t lds_nodes, int32_t flags,
+                                                           int32_t lds_nodes, int32_t lds_region, int32_t flags,
                                                            unsigned long long* stats) {
-    ORX_DYNAMIC_LDS(uint64_t, lds_keys);
+    ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(Ctl, ctl);
-    route_net<BLOCK>(g, P, nets, wave_nets[blockIdx.x], ws, ws_off[blockIdx.x], delta, lds_nodes, flags, stats,
-                     lds_keys, ctl);
+    route_net<BLOCK>(g, P, nets, wave_nets[blockIdx.x], ws, ws_off[blockIdx.x], delta, lds_nodes, lds_region, flags,
+                     stats, reinterpret_cast<uint32_t*>(lds_mem), ctl);
 }
 
 // Persistent dependency-driven form: workgroups claim nets in schedule order
@@ -557,9 +611,9 @@ __global__ __launch_bounds__(BLOCK) void route_dag_kernel(Lat g, Params P, NetsD
                                                           int32_t n, const int32_t* dep_ptr, const int32_t* dep_ids,
                                                           int32_t* done, int32_t epoch, uint32_t* next,
                                                           const int64_t* ws_off, orx_workspace ws, float delta,
-                                                          int32_t lds_nodes, int32_t flags,
+                                                          int32_t lds_nodes, int32_t lds_region, int32_t flags,
                                                           unsigned long long* stats) {
-    ORX_DYNAMIC_LDS(uint64_t, lds_keys);
+    ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(Ctl, ctl);
     ORX_SHARED(int32_t, s_pick);
     for (;;) {
@@ -576,7 +630,8 @@ __global__ __launch_bounds__(BLOCK) void route_dag_kernel(Lat g, Params P, NetsD
             }
         }
         __syncthreads();
-        route_net<BLOCK>(g, P, nets, k, ws, ws_off[k], delta, lds_nodes, flags, stats, lds_keys, ctl);
+        route_net<BLOCK>(g, P, nets, k, ws, ws_off[k], delta, lds_nodes, lds_region, flags, stats,
+                         reinterpret_cast<uint32_t*>(lds_mem), ctl);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_store(done + k, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -591,15 +646,15 @@ __global__ void zero_u32_kernel(uint32_t* p) {
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws, int32_t x0, int32_t y0, int32_t w,
                                                          int32_t h, int64_t src, int64_t dst, float delta,
-                                                         int32_t lds_nodes, int32_t flags, float* dist,
+                                                         int32_t lds_nodes, int32_t lds_region, int32_t flags,
+                                                         float* dist,
                                                          int32_t* parent, unsigned long long* stats,
                                                          int32_t* status) {
-    ORX_DYNAMIC_LDS(uint64_t, lds_keys);
+    ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(Ctl, ctl);
     const int32_t bx[4] = {x0, y0, w, h};
     const Box B = make_box(bx, g.L);
-    const bool use_lds = (int64_t)B.R <= (int64_t)lds_nodes;
-    const SlotState S = slot_state(ws, 0, use_lds, lds_keys, lds_nodes);
+    const SlotState S = slot_state(ws, 0, B.R, reinterpret_cast<uint32_t*>(lds_mem), lds_region, lds_nodes);
     if (threadIdx.x == 0) { ctl.settled = 0; ctl.relaxed = 0; ctl.rounds = 0; ctl.phases = 0; ctl.items = 0; }
     const uint32_t rsrc = global_to_roi(g, B, src);
     const bool full = (flags & ORX_F_FULL) != 0 || dst < 0;
@@ -610,7 +665,7 @@ __global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws
     }
     Heur H;
     H.on = false;
-    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.fA, S.fB, &ctl);
+    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA, S.fB, &ctl);
     if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
     for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
         const uint64_t k = key_load(S.keys + r);
@@ -1043,16 +1098,43 @@ int check_lattice(const orx_lattice* l) {
     return ORX_OK;
 }
 
-size_t lds_bytes_for(int32_t lds_nodes) {
-    return (size_t)lds_nodes * 8 + 3 * (size_t)((lds_nodes + 31) / 32) * 4;
+// Largest dynamic LDS a launch may ask for (160 KiB per CU, less the static
+// part of the kernels).
+constexpr size_t kMaxDynLds = 160 * 1024 - 256;
+
+// Dynamic LDS of a search launch whose largest ROI has max_nodes nodes: the
+// two near-queue rings plus a region big enough for the keys and bitmaps of
+// an ROI of min(lds_nodes, max_nodes) nodes or the bitmaps of the largest
+// ROI, whichever is larger, capped by the LDS of a CU (each ROI then keeps
+// in LDS what fits; see slot_state).
+struct LdsPlan {
+    size_t bytes;
+    int32_t region;
+    int32_t lds_nodes;
+};
+LdsPlan lds_plan(int64_t max_nodes, int32_t lds_nodes) {
+    if (lds_nodes < 0) lds_nodes = 0;
+    const int64_t ring = 2ll * kRing * 4;
+    const int64_t key_n = std::min<int64_t>(lds_nodes, max_nodes);
+    const int64_t key_need = key_n > 0 ? key_n * 8 + 3 * ((key_n + 31) / 32) * 4 : 0;
+    const int64_t bm_need = 3 * ((max_nodes + 31) / 32) * 4;
+    int64_t region = std::max(key_need, bm_need);
+    region = std::min<int64_t>(region, (int64_t)kMaxDynLds - ring) / 16 * 16;
+    return LdsPlan{(size_t)(ring + region), (int32_t)region, lds_nodes};
 }
 
-template <typename K>
-int set_lds(K kernel, size_t bytes) {
-    static size_t configured[8] = {0};
-    (void)configured;
-    ORX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+// Raise a kernel's dynamic-LDS limit once (orx_init sets every kernel to
+// kMaxDynLds up front, so launches -- also those recorded into a captured
+// graph -- make no attribute calls).
+template <auto KERNEL>
+int set_lds(size_t bytes) {
+    static std::atomic<size_t> configured{0};
+    if (bytes <= configured.load(std::memory_order_relaxed)) return ORX_OK;
+    ORX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(KERNEL),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    size_t cur = configured.load(std::memory_order_relaxed);
+    while (cur < bytes && !configured.compare_exchange_weak(cur, bytes)) {
+    }
     return ORX_OK;
 }
 
@@ -1159,7 +1241,59 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
     return rc;
 }
 
+// One route_wave_kernel launch (K nets, one workgroup each) after the
+// argument checks shared by orx_route_wave and orx_route_waves.
+int launch_route_wave(const Lat& g, const Params& P, const NetsDev& nd, const int32_t* d_wave_nets,
+                      const int64_t* d_ws_off, int32_t K, int64_t max_nodes, const orx_workspace* ws,
+                      const orx_route_cfg* cfg, hipStream_t s) {
+    if (K <= 0) return ORX_OK;
+    if (max_nodes < 1 || max_nodes >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI size outside [1, 2^24)");
+    const LdsPlan lp = lds_plan(max_nodes, cfg->lds_nodes);
+    const size_t lds = lp.bytes;
+    unsigned long long* st = (unsigned long long*)cfg->stats;
+    int rc = ORX_OK;
+#define ORX_LAUNCH_ROUTE(BS)                                                                              \
+    do {                                                                                                  \
+        rc = set_lds<route_wave_kernel<BS>>(lds);                                                         \
+        if (rc) return rc;                                                                                \
+        hipLaunchKernelGGL(route_wave_kernel<BS>, dim3(K), dim3(BS), lds, s, g, P, nd, d_wave_nets,       \
+                           d_ws_off, *ws, cfg->delta, lp.lds_nodes, lp.region, cfg->flags, st);           \
+    } while (0)
+    switch (cfg->block) {
+    case 256: ORX_LAUNCH_ROUTE(256); break;
+    case 512: ORX_LAUNCH_ROUTE(512); break;
+    case 1024: ORX_LAUNCH_ROUTE(1024); break;
+    default: return orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
+    }
+#undef ORX_LAUNCH_ROUTE
+    ORX_HIP_CHECK(hipGetLastError());
+    return ORX_OK;
+}
+
+NetsDev make_nets(const orx_nets* nets) {
+    return NetsDev{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
+                   nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+}
+
 }  // namespace
+
+namespace orx_host {
+
+int prepare_lattice_kernels() {
+    int rc = ORX_OK;
+    if (!rc) rc = set_lds<route_wave_kernel<256>>(kMaxDynLds);
+    if (!rc) rc = set_lds<route_wave_kernel<512>>(kMaxDynLds);
+    if (!rc) rc = set_lds<route_wave_kernel<1024>>(kMaxDynLds);
+    if (!rc) rc = set_lds<route_dag_kernel<256>>(kMaxDynLds);
+    if (!rc) rc = set_lds<route_dag_kernel<512>>(kMaxDynLds);
+    if (!rc) rc = set_lds<route_dag_kernel<1024>>(kMaxDynLds);
+    if (!rc) rc = set_lds<sssp_box_kernel<256>>(kMaxDynLds);
+    if (!rc) rc = set_lds<sssp_box_kernel<512>>(kMaxDynLds);
+    if (!rc) rc = set_lds<sssp_box_kernel<1024>>(kMaxDynLds);
+    return rc;
+}
+
+}  // namespace orx_host
 
 extern "C" {
 
@@ -1195,34 +1329,33 @@ int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* 
     if (rc) return rc;
     if (!p || !nets || !ws || !cfg || !d_wave_nets || !d_ws_off) return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (K <= 0) return ORX_OK;
-    if (max_nodes >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI larger than 2^24 nodes");
     if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
-    int32_t lds_nodes = cfg->lds_nodes;
-    if (lds_nodes < 0) lds_nodes = 0;
-    if ((int64_t)lds_nodes > max_nodes) lds_nodes = (int32_t)((max_nodes + 63) / 64 * 64);
-    const size_t lds = lds_bytes_for(lds_nodes);
-    if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
-    NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
-               nets->path_cap, nets->path_len, nets->path_cost, nets->status};
-    unsigned long long* st = (unsigned long long*)cfg->stats;
-    const hipStream_t s = (hipStream_t)stream;
+    return launch_route_wave(make_lat(lat), make_params(p), make_nets(nets), d_wave_nets, d_ws_off, K, max_nodes, ws,
+                             cfg, (hipStream_t)stream);
+}
+
+int orx_route_waves(const orx_lattice* lat, const orx_params* p, const orx_nets* nets, const int32_t* d_nets,
+                    const int64_t* d_ws_off, int32_t n_waves, const int32_t* h_wave_ptr, const int64_t* h_wave_max,
+                    const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream) {
+    int rc = check_lattice(lat);
+    if (rc) return rc;
+    if (!p || !nets || !ws || !cfg || !h_wave_ptr || !h_wave_max) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (n_waves <= 0) return ORX_OK;
+    if (!d_nets || !d_ws_off) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
+    if (h_wave_ptr[0] != 0) return orx_host::fail(ORX_E_ARG, "wave_ptr[0] must be 0");
+    for (int32_t i = 0; i < n_waves; ++i)
+        if (h_wave_ptr[i + 1] < h_wave_ptr[i] || h_wave_ptr[i + 1] > nets->n)
+            return orx_host::fail(ORX_E_ARG, "wave_ptr must be non-decreasing and at most n");
     const Lat g = make_lat(lat);
     const Params P = make_params(p);
-#define ORX_LAUNCH_ROUTE(BS)                                                                              \
-    do {                                                                                                  \
-        rc = set_lds(route_wave_kernel<BS>, lds);                                                         \
-        if (rc) return rc;                                                                                \
-        hipLaunchKernelGGL(route_wave_kernel<BS>, dim3(K), dim3(BS), lds, s, g, P, nd, d_wave_nets,       \
-                           d_ws_off, *ws, cfg->delta, lds_nodes, cfg->flags, st);                         \
-    } while (0)
-    switch (cfg->block) {
-    case 256: ORX_LAUNCH_ROUTE(256); break;
-    case 512: ORX_LAUNCH_ROUTE(512); break;
-    case 1024: ORX_LAUNCH_ROUTE(1024); break;
-    default: return orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
+    const NetsDev nd = make_nets(nets);
+    const hipStream_t s = (hipStream_t)stream;
+    for (int32_t i = 0; i < n_waves; ++i) {
+        const int32_t a = h_wave_ptr[i], K = h_wave_ptr[i + 1] - a;
+        rc = launch_route_wave(g, P, nd, d_nets + a, d_ws_off + a, K, h_wave_max[i], ws, cfg, s);
+        if (rc) return rc;
     }
-#undef ORX_LAUNCH_ROUTE
-    ORX_HIP_CHECK(hipGetLastError());
     return ORX_OK;
 }
 
@@ -1245,10 +1378,8 @@ int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* n
     if (n <= 0) return ORX_OK;
     if (max_nodes >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI larger than 2^24 nodes");
     if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
-    int32_t lds_nodes = cfg->lds_nodes < 0 ? 0 : cfg->lds_nodes;
-    if ((int64_t)lds_nodes > max_nodes) lds_nodes = (int32_t)((max_nodes + 63) / 64 * 64);
-    const size_t lds = lds_bytes_for(lds_nodes);
-    if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
+    const LdsPlan lp = lds_plan(max_nodes, cfg->lds_nodes);
+    const size_t lds = lp.bytes;
     NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
                nets->path_cap, nets->path_len, nets->path_cost, nets->status};
     unsigned long long* st = (unsigned long long*)cfg->stats;
@@ -1261,7 +1392,7 @@ int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* n
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, d_counter);
 #define ORX_LAUNCH_DAG(BS)                                                                                \
     do {                                                                                                  \
-        rc = set_lds(route_dag_kernel<BS>, lds);                                                          \
+        rc = set_lds<route_dag_kernel<BS>>(lds);                                                          \
         if (rc) return rc;                                                                                \
         int per_cu = 0;                                                                                   \
         ORX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, route_dag_kernel<BS>, BS, lds)); \
@@ -1270,7 +1401,7 @@ int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* n
         grid = std::min(grid, n);                                                                         \
         hipLaunchKernelGGL(route_dag_kernel<BS>, dim3(grid), dim3(BS), lds, s, g, P, nd, d_sched, n,      \
                            d_dep_ptr, d_dep_ids, d_done, epoch, d_counter, d_ws_off, *ws, cfg->delta,     \
-                           lds_nodes, cfg->flags, st);                                                    \
+                           lp.lds_nodes, lp.region, cfg->flags, st);                                      \
     } while (0)
     switch (cfg->block) {
     case 256: ORX_LAUNCH_DAG(256); break;
@@ -1339,20 +1470,18 @@ int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int3
     if (use_grid) return sssp_box_grid(g, x0, y0, w, h, L_of(lat), (uint32_t)to_roi(src),
                                        dst >= 0 ? (uint32_t)to_roi(dst) : 0u, dst < 0 || (cfg->flags & ORX_F_FULL),
                                        d_dist, d_parent, ws, cfg, s);
-    int32_t lds_nodes = cfg->lds_nodes < 0 ? 0 : cfg->lds_nodes;
-    if ((int64_t)lds_nodes > R) lds_nodes = (int32_t)((R + 63) / 64 * 64);
-    const size_t lds = lds_bytes_for(lds_nodes);
-    if (lds > 160 * 1024 - 256) return orx_host::fail(ORX_E_ARG, "lds_nodes exceeds the 160 KiB LDS of a CU");
+    const LdsPlan lp = lds_plan(R, cfg->lds_nodes);
+    const size_t lds = lp.bytes;
     int32_t* d_status = nullptr;
     int32_t h_status = 0;
     ORX_HIP_CHECK(hipMallocAsync((void**)&d_status, sizeof(int32_t), s));
     int rc = ORX_OK;
 #define ORX_LAUNCH_BOX(BS)                                                                                 \
     do {                                                                                                   \
-        rc = set_lds(sssp_box_kernel<BS>, lds);                                                            \
+        rc = set_lds<sssp_box_kernel<BS>>(lds);                                                            \
         if (rc) break;                                                                                     \
         hipLaunchKernelGGL(sssp_box_kernel<BS>, dim3(1), dim3(BS), lds, s, g, *ws, x0, y0, w, h, src, dst, \
-                           cfg->delta, lds_nodes, cfg->flags, d_dist, d_parent,                            \
+                           cfg->delta, lp.lds_nodes, lp.region, cfg->flags, d_dist, d_parent,              \
                            (unsigned long long*)cfg->stats, d_status);                                     \
     } while (0)
     switch (cfg->block) {

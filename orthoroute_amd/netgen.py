@@ -44,6 +44,12 @@ class Netlist:
         return Netlist(self.spec, self.src[:k].copy(), self.dst[:k].copy(), self.box[:k].copy(),
                        f"{self.name}[:{k}]", dict(self.meta))
 
+    def select(self, idx, tag: str = "") -> "Netlist":
+        """The nets ``idx`` (in that order) on the same board."""
+        idx = np.asarray(idx, np.int64)
+        return Netlist(self.spec, self.src[idx].copy(), self.dst[idx].copy(), self.box[idx].copy(),
+                       f"{self.name}[{tag or len(idx)}]", dict(self.meta))
+
 
 def roi_boxes(spec: LatticeSpec, src: np.ndarray, dst: np.ndarray, margin: int) -> np.ndarray:
     """Bounding box of the two pins plus ``margin`` cells, clipped to the board."""

@@ -30,8 +30,8 @@ import torch
 from . import _lib
 from .lattice import LatticeSpec
 from .netgen import Netlist
-from .waves import (check_schedule, color_waves, group_waves, net_distance, net_order, plan_iteration,
-                    pres_schedule, resolve_pres_fac_max)
+from .waves import (check_schedule, net_distance, net_order, plan_iteration, plan_waves, pres_schedule,
+                    resolve_pres_fac_max)
 
 
 @dataclass
@@ -57,6 +57,8 @@ class RouterConfig:
     astar: bool = True
     stop_at_zero_overuse: bool = False
     collect_stats: bool = True
+    graph: bool = False               # capture each iteration's wave launches in one hipGraph
+    ws_cap_nodes: int = 1 << 24       # search workspace bound: larger waves launch in parts
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -273,6 +275,80 @@ def _wave_offsets(roi_nodes: np.ndarray, nets: np.ndarray) -> tuple[np.ndarray, 
     return off, int(r.sum())
 
 
+def split_waves(waves: list[np.ndarray], roi_nodes: np.ndarray, cap_nodes: int) -> list[np.ndarray]:
+    """Split every wave whose 64-aligned ROI total exceeds ``cap_nodes`` into
+    consecutive parts that fit (nets of one wave are independent, so any
+    split gives the same result).  Every ROI must fit on its own."""
+    out = []
+    for w in waves:
+        r64 = (roi_nodes[w].astype(np.int64) + 63) // 64 * 64
+        if len(w) == 0 or int(r64.sum()) <= cap_nodes:
+            out.append(w)
+            continue
+        if int(r64.max()) > cap_nodes:
+            raise ValueError("an ROI exceeds the workspace cap")
+        start, acc = 0, 0
+        for i, r in enumerate(r64.tolist()):
+            if acc + r > cap_nodes:
+                out.append(w[start:i])
+                start, acc = i, 0
+            acc += r
+        out.append(w[start:])
+    return out
+
+
+def footprint_estimate(netlist: Netlist, cfg: RouterConfig | None = None) -> int:
+    """HBM bytes a DevicePathFinder on ``netlist`` holds at most: lattice
+    arrays (72 B/node + 4 B/column), path arena (4 B per ROI node of every
+    net), per-net tables, and the search workspace of the largest launch
+    (24.375 B per node, bounded by the workspace cap)."""
+    cfg = cfg or RouterConfig()
+    spec = netlist.spec
+    R = netlist.roi_nodes().astype(np.int64)
+    r64 = (R + 63) // 64 * 64
+    ws_nodes = min(int(r64.sum()), max(int(cfg.ws_cap_nodes), int(r64.max(initial=64))))
+    ws_nodes = int(ws_nodes * 1.25) // 64 * 64 + 64  # Workspace.ensure grows by 1.25x
+    lattice = spec.N * 72 + spec.XY * 4
+    nets = int(R.sum()) * 4 + netlist.n * 64
+    return lattice + nets + ws_nodes * 24 + 3 * ws_nodes // 8 + (1 << 20)
+
+
+def _flatten_waves(waves: list[np.ndarray], roi_nodes: np.ndarray):
+    """Launch plan of orx_route_waves: net ids and 64-aligned workspace
+    offsets (restarting at 0 in every wave), wave_ptr[n_waves+1],
+    wave_max[n_waves] (largest ROI of each wave) and the workspace size the
+    largest wave needs."""
+    nw = len(waves)
+    sizes = np.fromiter((len(w) for w in waves), np.int64, nw)
+    wave_ptr = np.zeros(nw + 1, np.int32)
+    np.cumsum(sizes, out=wave_ptr[1:])
+    if nw == 0:
+        return (np.zeros(0, np.int32), np.zeros(0, np.int64), wave_ptr, np.zeros(0, np.int64), 0)
+    flat = np.concatenate(waves).astype(np.int32)
+    R = roi_nodes[flat].astype(np.int64)
+    r64 = (R + 63) // 64 * 64
+    incl = np.cumsum(r64)
+    start = np.repeat(incl[wave_ptr[:-1]] - r64[wave_ptr[:-1]], sizes)
+    off = (incl - r64 - start).astype(np.int64)
+    wave_max = np.maximum.reduceat(R, wave_ptr[:-1]) if len(flat) else np.zeros(nw, np.int64)
+    totals = np.add.reduceat(r64, wave_ptr[:-1])
+    return flat, off, wave_ptr, wave_max.astype(np.int64), int(totals.max())
+
+
+@dataclass
+class _Pending:
+    """An iteration enqueued by launch_iteration and not yet collected."""
+    it: int
+    pres: float
+    waves: int
+    t0: float
+    events: tuple
+    done: torch.cuda.Event
+    graph: object
+    keep: tuple        # device buffers the launches read
+    nets: int
+
+
 class DevicePathFinder:
     """PathFinder on one routing job, state resident on the GPU."""
 
@@ -283,20 +359,28 @@ class DevicePathFinder:
         self.cfg = cfg or RouterConfig()
         self.spec = netlist.spec
         self.device = torch.device(device)
-        self.stream = stream
-        self.lat = LatticeState(self.spec, base, device)
-        self.nets = NetTables(netlist, device)
-        self.ws = Workspace(device)
+        # every torch op and kernel of this job runs on one stream, so jobs on
+        # different streams overlap on the device and never race
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        if base is None:
+            base = self.spec.base_costs()
+        with torch.cuda.stream(self.stream):
+            self.lat = LatticeState(self.spec, base, device)
+            self.nets = NetTables(netlist, device)
+            self.ws = Workspace(device)
+            self._route_stats = torch.zeros(8, dtype=torch.int64, device=self.device)
+            self._iter_out = torch.zeros(8, dtype=torch.int64, device=self.device)
+            self._over = torch.zeros(max(1, netlist.n), dtype=torch.int64, device=self.device)
+            self._hops = torch.zeros(max(1, netlist.n), dtype=torch.int32, device=self.device)
+        self._iter_out_h = torch.zeros(8, dtype=torch.int64, pin_memory=True)
+        self._route_stats_h = torch.zeros(8, dtype=torch.int64, pin_memory=True)
         self.distance = net_distance(self.spec, netlist.src, netlist.dst)
         self.schedule = self.cfg.schedule(self.spec.L)
-        self.astar = self.cfg.astar_bounds(self.spec.base_costs() if base is None else base)
+        self.astar = self.cfg.astar_bounds(base)
         self.iteration = 0
         self.max_groups = 0
         self.history: list[IterationStats] = []
-        self._route_stats = torch.zeros(8, dtype=torch.int64, device=self.device)
-        self._iter_out = torch.zeros(8, dtype=torch.int64, device=self.device)
-        self._over = torch.zeros(max(1, netlist.n), dtype=torch.int64, device=self.device)
-        self._hops = torch.zeros(max(1, netlist.n), dtype=torch.int32, device=self.device)
+        self._pending: _Pending | None = None
 
     # -- helpers -----------------------------------------------------------
     def _s(self) -> int:
@@ -304,6 +388,34 @@ class DevicePathFinder:
 
     def footprint_bytes(self) -> int:
         return self.lat.nbytes() + self.nets.nbytes() + self.ws.nbytes()
+
+    _STATE = (("lat", ("hist", "occ", "wcost", "node_hist", "node_use", "via_col")),
+              ("nets", ("path_arena", "path_len", "path_cost", "status")))
+
+    def snapshot(self) -> dict:
+        """Device copy of every array an iteration mutates, plus the host
+        iteration counter (for re-running an iteration, e.g. to time it
+        under other launch parameters)."""
+        if self._pending is not None:
+            raise RuntimeError("an iteration is in flight")
+        snap = {"iteration": self.iteration, "history": len(self.history)}
+        with torch.cuda.stream(self.stream):
+            for owner, names in self._STATE:
+                obj = getattr(self, owner)
+                for n in names:
+                    snap[(owner, n)] = getattr(obj, n).clone()
+        return snap
+
+    def restore(self, snap: dict) -> None:
+        if self._pending is not None:
+            raise RuntimeError("an iteration is in flight")
+        with torch.cuda.stream(self.stream):
+            for owner, names in self._STATE:
+                obj = getattr(self, owner)
+                for n in names:
+                    getattr(obj, n).copy_(snap[(owner, n)])
+        self.iteration = snap["iteration"]
+        del self.history[snap["history"]:]
 
     def route_cfg(self, flags: int = 0) -> _lib.OrxRouteCfg:
         c = self.cfg
@@ -314,88 +426,131 @@ class DevicePathFinder:
         """_order_nets_by_difficulty from the device occupancy (one readback)."""
         n = self.netlist.n
         if iteration > 1 and n:
-            _lib.call("orx_net_overuse", C.byref(self.lat.struct), C.byref(self.nets.struct),
-                      self.cfg.edge_cap, self._over.data_ptr(), self._hops.data_ptr(), self._s())
-            over = self._over[:n].cpu().numpy()
-            hops = self._hops[:n].cpu().numpy()
+            with torch.cuda.stream(self.stream):
+                _lib.call("orx_net_overuse", C.byref(self.lat.struct), C.byref(self.nets.struct),
+                          self.cfg.edge_cap, self._over.data_ptr(), self._hops.data_ptr(), self._s())
+                over = self._over[:n].cpu().numpy()
+                hops = self._hops[:n].cpu().numpy()
         else:
             over = np.zeros(n, np.int64)
             hops = np.zeros(n, np.int32)
         return net_order(iteration, self.distance, over, hops)
 
     def plan(self, order: np.ndarray, mode: str) -> list[np.ndarray]:
-        """Launch groups: one per net (plain) or one per disjoint-ROI wave."""
+        """Launch groups: one per net (plain) or one per disjoint-ROI wave
+        (native planner; inside a wave the largest ROI comes first)."""
         if mode == "plain":
             return [np.asarray([k], np.int32) for k in order]
         if mode not in ("batched", "waves"):
             raise ValueError("mode must be 'batched' ('waves') or 'plain'")
-        wave = color_waves(self.netlist.box, order, self.spec.X, self.spec.Y)
-        return group_waves(wave, order, self.nets.roi_nodes)
+        waves = plan_waves(self.netlist.box, order, self.spec.X, self.spec.Y, self.nets.roi_nodes)
+        return split_waves(waves, self.nets.roi_nodes, self.ws_cap())
+
+    def ws_cap(self) -> int:
+        """Workspace nodes one launch may use (at least the largest ROI)."""
+        r64 = (int(self.nets.roi_nodes.max(initial=1)) + 63) // 64 * 64
+        return max(int(self.cfg.ws_cap_nodes), r64)
 
     # -- one iteration -----------------------------------------------------
     def run_iteration(self, mode: str = "batched", order: np.ndarray | None = None,
-                      waves: list[np.ndarray] | None = None) -> IterationStats:
+                      waves: list[np.ndarray] | None = None, graph: bool | None = None) -> IterationStats:
+        """One PathFinder iteration, synchronous."""
+        return self.finish_iteration(self.launch_iteration(mode, order, waves, graph))
+
+    def launch_iteration(self, mode: str = "batched", order: np.ndarray | None = None,
+                         waves: list[np.ndarray] | None = None, graph: bool | None = None) -> "_Pending":
+        """Enqueue one iteration on this job's stream and return without
+        waiting for it: re-price, the waves (one launch each, or one captured
+        hipGraph of them when ``graph``), then overuse + history.  The host
+        work (order readback, planning, upload) happens here; the device work
+        overlaps whatever other streams run.  finish_iteration collects it."""
+        if self._pending is not None:
+            raise RuntimeError("an iteration of this job is still in flight; call finish_iteration first")
         if self.iteration >= len(self.schedule):
             raise RuntimeError("schedule exhausted; raise cfg.iterations")
+        if mode == "dag":
+            raise RuntimeError("the dependency-driven launch (orx_route_dag) is switched off on the device")
+        graph = self.cfg.graph if graph is None else graph
         it = self.iteration + 1
         pres = float(self.schedule[it - 1])
         params = self.cfg.params(pres, self.astar)
-        stream = self.stream if self.stream is not None else torch.cuda.current_stream()
+        stream = self.stream
         t0 = time.perf_counter()
         if order is None:
             order = self.order_for(it)
-        if mode == "dag":
-            raise RuntimeError("the dependency-driven launch (orx_route_dag) is switched off on the device")
         if waves is None:
             waves = self.plan(order, mode)
-        # upload the wave plan once
-        offs = []
-        need = 0
-        for w in waves:
-            off, total = _wave_offsets(self.nets.roi_nodes, w)
-            offs.append(off)
-            need = max(need, total)
-        self.ws.ensure(need)
-        flat_nets = np.concatenate(waves) if waves else np.zeros(0, np.int32)
-        flat_off = np.concatenate(offs) if offs else np.zeros(0, np.int64)
-        d_nets = torch.from_numpy(flat_nets.astype(np.int32)).to(self.device, non_blocking=False)
-        d_off = torch.from_numpy(flat_off.astype(np.int64)).to(self.device, non_blocking=False)
-        self._route_stats.zero_()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        evr0 = torch.cuda.Event(enable_timing=True)
-        evr1 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-        self.lat.reprice(params, stream)
-        rcfg = self.route_cfg()
-        evr0.record(stream)
+        flat_nets, flat_off, wave_ptr, wave_max, need = _flatten_waves(waves, self.nets.roi_nodes)
         s = self._s()
-        pos = 0
-        for w in waves:
-            K = len(w)
-            maxR = int(self.nets.roi_nodes[w].max())
-            _lib.call("orx_route_wave", C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
-                      d_nets.data_ptr() + 4 * pos, d_off.data_ptr() + 8 * pos, K, maxR,
-                      C.byref(self.ws.struct), C.byref(rcfg), s)
-            pos += K
-        evr1.record(stream)
-        _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
-                  self._iter_out.data_ptr(), s)
-        ev1.record(stream)
-        out = self._iter_out.cpu().numpy()
-        rs = self._route_stats.cpu().numpy()
-        stream.synchronize()
-        dt = time.perf_counter() - t0
+        ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(4))
+        with torch.cuda.stream(stream):
+            self.ws.ensure(need)
+            d_nets = torch.from_numpy(flat_nets).to(self.device)
+            d_off = torch.from_numpy(flat_off).to(self.device)
+            self._route_stats.zero_()
+            ev[0].record(stream)
+            self.lat.reprice(params, stream)
+            rcfg = self.route_cfg()
+            ev[1].record(stream)
+            exec_graph = None
+            route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct), d_nets.data_ptr(),
+                          d_off.data_ptr(), len(waves), wave_ptr.ctypes.data, wave_max.ctypes.data,
+                          C.byref(self.ws.struct), C.byref(rcfg))
+            if graph and len(waves):
+                cs = C.c_void_p()
+                _lib.call("orx_graph_begin", C.byref(cs))
+                try:
+                    _lib.call("orx_route_waves", *route_args, cs)
+                finally:
+                    g = C.c_void_p()
+                    _lib.call("orx_graph_end", cs, C.byref(g))
+                exec_graph = g
+                _lib.call("orx_graph_launch", exec_graph, s)
+            else:
+                _lib.call("orx_route_waves", *route_args, s)
+            ev[2].record(stream)
+            _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
+                      self._iter_out.data_ptr(), s)
+            ev[3].record(stream)
+            self._iter_out_h.copy_(self._iter_out, non_blocking=True)
+            self._route_stats_h.copy_(self._route_stats, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(stream)
+        self._pending = _Pending(it, pres, len(waves), t0, ev, done, exec_graph, (d_nets, d_off),
+                                 int(sum(len(w) for w in waves)))
+        return self._pending
+
+    def ready(self) -> bool:
+        """True when the iteration in flight (if any) has finished on the device."""
+        return self._pending is None or self._pending.done.query()
+
+    def finish_iteration(self, pending: "_Pending | None" = None) -> IterationStats:
+        p = pending or self._pending
+        if p is None or p is not self._pending:
+            raise RuntimeError("no iteration of this job in flight")
+        p.done.synchronize()
+        if p.graph is not None:
+            _lib.call("orx_graph_destroy", p.graph)
+        dt = time.perf_counter() - p.t0
+        out = self._iter_out_h.numpy().copy()
+        rs = self._route_stats_h.numpy().copy()
+        ev0, evr0, evr1, ev1 = p.events
         st = IterationStats(
-            iteration=it, pres_fac=pres, routed=int(rs[5]), failed=int(rs[6]),
+            iteration=p.it, pres_fac=p.pres, routed=int(rs[5]), failed=int(rs[6]),
             edge_over=int(out[0]), edge_over_count=int(out[1]), node_over=int(out[2]),
             node_over_count=int(out[3]), via_hops=int(out[4]), barrels=int(out[5]),
             settled=int(rs[0]), relaxed=int(rs[1]), items=int(rs[2]), rounds=int(rs[3]), phases=int(rs[4]),
-            path_hops=int(rs[7]), waves=len(waves), seconds=dt, gpu_ms=ev0.elapsed_time(ev1),
-            route_ms=evr0.elapsed_time(evr1))
-        self.iteration = it
+            path_hops=int(rs[7]), waves=p.waves, seconds=dt, gpu_ms=ev0.elapsed_time(ev1),
+            route_ms=evr0.elapsed_time(evr1), extra={"graph": p.graph is not None, "launched_nets": p.nets})
+        self._pending = None
+        self.iteration = p.it
         self.history.append(st)
         return st
+
+    @property
+    def converged(self) -> bool:
+        """Zero overuse and no failed net after the last iteration."""
+        return bool(self.history) and self.history[-1].overuse == 0 and self.history[-1].failed == 0
 
     def _run_dag(self, it, pres, params, order, stream, t0) -> IterationStats:
         """One persistent dependency-driven launch for the whole iteration."""
@@ -449,10 +604,32 @@ class DevicePathFinder:
 
     # -- results -------------------------------------------------------------
     def paths(self) -> list[np.ndarray]:
-        return self.nets.paths()
+        with torch.cuda.stream(self.stream):
+            return self.nets.paths()
+
+    def digest(self) -> str:
+        """SHA-256 over every net's path (node ids), its fp32 cost and the
+        final occupancy: equal digests mean bit-identical results."""
+        import hashlib
+        h = hashlib.sha256()
+        with torch.cuda.stream(self.stream):
+            lens = self.nets.path_len.cpu().numpy()
+            arena = self.nets.path_arena.cpu().numpy()
+            cost = self.nets.path_cost.cpu().numpy()
+            occ = self.lat.occ.cpu().numpy()
+            use = self.lat.node_use.cpu().numpy()
+        for o, l in zip(self.nets.path_off_np, lens):
+            h.update(arena[o:o + l].tobytes())
+            h.update(b"|")
+        h.update(cost.tobytes())
+        h.update(occ.tobytes())
+        h.update(use.tobytes())
+        return h.hexdigest()
 
     def path_costs(self) -> np.ndarray:
-        return self.nets.path_cost.cpu().numpy()
+        with torch.cuda.stream(self.stream):
+            return self.nets.path_cost.cpu().numpy()
 
     def status(self) -> np.ndarray:
-        return self.nets.status.cpu().numpy()
+        with torch.cuda.stream(self.stream):
+            return self.nets.status.cpu().numpy()

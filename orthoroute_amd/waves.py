@@ -125,6 +125,34 @@ def plan_iteration(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes
         cap = int(total)
 
 
+def plan_waves(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes: np.ndarray) -> list[np.ndarray]:
+    """The disjoint-ROI waves of one iteration from the native planner: the
+    same waves and in-wave order as group_waves(color_waves(...)), computed
+    in C++ (no dependency lists)."""
+    import ctypes as C
+
+    from . import _lib
+    lib = _lib.load()
+    n = len(order)
+    if n == 0:
+        return []
+    box = np.ascontiguousarray(box.reshape(-1), np.int32)
+    order = np.ascontiguousarray(order, np.int32)
+    rn = np.ascontiguousarray(roi_nodes, np.int64)
+    wave = np.empty(n, np.int32)
+    sched = np.empty(n, np.int32)
+    dep_ptr = np.empty(n + 1, np.int32)
+    dep_ids = np.empty(1, np.int32)
+    total = lib.orx_plan_iteration(box.ctypes.data, order.ctypes.data, n, X, Y, rn.ctypes.data,
+                                   wave.ctypes.data, sched.ctypes.data, dep_ptr.ctypes.data,
+                                   dep_ids.ctypes.data, 0)
+    if total < 0:
+        raise ValueError(f"orx_plan_iteration rejected its input ({total})")
+    ws = wave[sched]
+    cuts = np.flatnonzero(np.diff(ws)) + 1
+    return np.split(sched, cuts)
+
+
 def check_schedule(sched: np.ndarray, dep_ptr: np.ndarray, dep_ids: np.ndarray) -> None:
     """Every dependency must come earlier in the schedule (the launch's
     progress guarantee rests on it)."""

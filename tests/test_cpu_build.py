@@ -38,3 +38,58 @@ def test_route_kernels_on_host_threads(driver, tmp_path, monkeypatch, mode, grou
             assert np.array_equal(a, b)
         assert np.array_equal(c.occ, o.occ)
         assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
+
+
+def _bfs_layer_max(spec, box, src):
+    """Largest hop layer of a breadth-first search from src inside the box."""
+    x0, y0, w, h = box
+    nb = spec.neighbours()
+    x, y, z = spec.coords(np.arange(spec.N))
+    inside = (x >= x0) & (x < x0 + w) & (y >= y0) & (y < y0 + h)
+    seen = np.zeros(spec.N, bool)
+    seen[src] = True
+    layer = np.array([src])
+    best = 1
+    while len(layer):
+        cand = nb[layer].reshape(-1)
+        cand = np.unique(cand[cand >= 0])
+        cand = cand[inside[cand] & ~seen[cand]]
+        seen[cand] = True
+        layer = cand
+        best = max(best, len(layer))
+    return best
+
+
+def test_box_search_near_queue_spills_past_the_lds_ring(driver, tmp_path):
+    """A pure frontier search (delta = inf) whose rounds hold more nodes than
+    the 2048-entry LDS ring of the near queue: dist and parent arrays equal
+    the oracle's, with every access in bounds."""
+    from oracle.oracle import sssp_box as oracle_sssp_box
+    from orthoroute_amd.lattice import LatticeSpec
+    from tests.cpu_build.harness import sssp_box_host_threads
+    spec = LatticeSpec(184, 182, 8)
+    w = spec.base_costs()
+    box = (2, 1, 180, 180)
+    src = int(spec.node(92, 91, 0))
+    assert _bfs_layer_max(spec, box, src) > 2048
+    dist, parent, stats = sssp_box_host_threads(spec, w, box, src, -1, tmp_path, grid=False, delta=float("inf"),
+                                                block=256, timeout=900)
+    od, op, _, _, _ = oracle_sssp_box(spec, w, box, src)
+    assert np.array_equal(dist.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(parent, op)
+
+
+def test_grid_search_on_host_threads(driver, tmp_path):
+    """The grid-wide state machine (captured step graph replayed until done)
+    on host threads: bit-exact with the oracle."""
+    from oracle.oracle import sssp_box as oracle_sssp_box
+    from orthoroute_amd.lattice import LatticeSpec
+    from tests.cpu_build.harness import sssp_box_host_threads
+    spec = LatticeSpec(26, 20, 3)
+    w = spec.random_edge_costs(9)
+    box = (3, 2, 20, 16)
+    src = int(spec.node(10, 10, 0))
+    dist, parent, stats = sssp_box_host_threads(spec, w, box, src, -1, tmp_path, grid=True, delta=4.0, block=256)
+    od, op, _, _, _ = oracle_sssp_box(spec, w, box, src)
+    assert np.array_equal(dist.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(parent, op)

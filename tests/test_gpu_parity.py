@@ -70,6 +70,14 @@ def test_uneven_small_bit_exact(device, lds_nodes, delta, astar):
     _run_both(nl, cfg, 4, "batched", device)
 
 
+@pytest.mark.parametrize("lds_nodes", [0, 16384])
+def test_wide_bucket_large_block_bit_exact(device, lds_nodes):
+    """The tuned launch shape (wide bucket, 1024 lanes) on a congested board."""
+    nl = uneven(96, 80, 8, 300, seed=12)
+    cfg = RouterConfig(iterations=3, delta=16.0, block=1024, lds_nodes=lds_nodes)
+    _run_both(nl, cfg, 3, "batched", device)
+
+
 @pytest.mark.parametrize("block", [256, 512, 1024])
 def test_block_sizes_bit_exact(device, block):
     nl = uneven(64, 64, 6, 150, seed=5)

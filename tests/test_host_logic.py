@@ -1,0 +1,115 @@
+"""CPU tests of the host side: wave planning, launch-plan flattening, the
+present-factor schedule, net order, and the C ABI's exported symbols (the
+library loads without a GPU; no compute call is made)."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from orthoroute_amd import _lib
+from orthoroute_amd.netgen import backplane_400, plain_64, uneven
+from orthoroute_amd.router import _flatten_waves
+from orthoroute_amd.waves import (check_disjoint, color_waves, group_waves, net_distance, net_order, plan_waves,
+                                  pres_schedule, resolve_pres_fac_max)
+
+REPO = Path(__file__).resolve().parent.parent
+
+
+def _boards():
+    yield plain_64()
+    yield uneven(96, 80, 6, 300, seed=5)
+    bp = backplane_400()
+    yield bp.select(np.arange(0, bp.n, 7))
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_native_waves_equal_python_colouring(k):
+    nl = list(_boards())[k]
+    rng = np.random.default_rng(k)
+    order = rng.permutation(nl.n).astype(np.int32)
+    R = nl.roi_nodes()
+    want = group_waves(color_waves(nl.box, order, nl.spec.X, nl.spec.Y), order, R)
+    got = plan_waves(nl.box, order, nl.spec.X, nl.spec.Y, R)
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+def test_waves_are_disjoint_and_keep_sequential_order():
+    nl = uneven(120, 120, 4, 400, seed=11)
+    order = np.random.default_rng(3).permutation(nl.n).astype(np.int32)
+    waves = plan_waves(nl.box, order, nl.spec.X, nl.spec.Y, nl.roi_nodes())
+    assert sorted(np.concatenate(waves).tolist()) == list(range(nl.n))
+    wave_of = np.empty(nl.n, np.int64)
+    for i, w in enumerate(waves):
+        assert check_disjoint(nl.box, w)
+        wave_of[w] = i
+    pos = np.empty(nl.n, np.int64)
+    pos[order] = np.arange(nl.n)
+    b = nl.box.astype(np.int64)
+    x0, y0, x1, y1 = b[:, 0], b[:, 1], b[:, 0] + b[:, 2], b[:, 1] + b[:, 3]
+    ov = (x0[:, None] < x1[None]) & (x0[None] < x1[:, None]) & (y0[:, None] < y1[None]) & (y0[None] < y1[:, None])
+    np.fill_diagonal(ov, False)
+    i, j = np.nonzero(ov & (pos[:, None] < pos[None, :]))
+    # an overlapping pair routes in routing order: the earlier net in an earlier wave
+    assert (wave_of[i] < wave_of[j]).all()
+
+
+def test_flatten_waves_offsets_and_sizes():
+    R = np.array([10, 64, 65, 1, 200, 7], np.int64)
+    waves = [np.array([2, 0], np.int32), np.array([4], np.int32), np.array([1, 5, 3], np.int32)]
+    flat, off, ptr, wmax, need = _flatten_waves(waves, R)
+    assert flat.tolist() == [2, 0, 4, 1, 5, 3]
+    assert ptr.tolist() == [0, 2, 3, 6]
+    assert off.tolist() == [0, 128, 0, 0, 64, 128]
+    assert wmax.tolist() == [65, 200, 64]
+    assert need == max(128 + 64, 256, 64 + 64 + 64)
+    assert (off % 64 == 0).all()
+    e = _flatten_waves([], R)
+    assert e[2].tolist() == [0] and e[4] == 0
+
+
+def test_pres_schedule_is_exact_fp32_and_capped():
+    s = pres_schedule(12, 0.5, 1.5, 32.0)
+    assert s.dtype == np.float32
+    want = [0.5 * 1.5 ** k for k in range(12)]
+    for got, w in zip(s, want):
+        assert float(got) == min(w, 32.0)  # dyadic rationals: exact while 3^k < 2^24
+    assert resolve_pres_fac_max(6) == 64.0          # configured 64 beats the 4-signal-layer floor
+    assert resolve_pres_fac_max(24, configured=1.0) == 128.0
+
+
+def test_net_order_hardest_first_with_rotation():
+    nl = plain_64()
+    d = net_distance(nl.spec, nl.src, nl.dst)
+    o1 = net_order(1, d, np.zeros(nl.n, np.int64), np.zeros(nl.n, np.int32))
+    assert sorted(o1.tolist()) == list(range(nl.n))
+    assert np.array_equal(o1, net_order(1, d, np.zeros(nl.n, np.int64), np.zeros(nl.n, np.int32)))
+    # congestion raises difficulty: a net with overuse moves towards the front
+    over = np.zeros(nl.n, np.int64)
+    easiest = int(np.argmin(np.where(d > 0, d, d.max() + 1)))
+    over[easiest] = 10 ** 6
+    hops = np.ones(nl.n, np.int32)
+    o2 = net_order(2, d, over, hops)
+    # hardest first, then the whole order rotated left by random.Random(42 + it)
+    import random
+    rot = random.Random(44).randint(0, min(5, nl.n // 10))
+    assert list(o2).index(easiest) == (nl.n - rot) % nl.n
+
+
+def _declared_functions():
+    text = (REPO / "include" / "orthoroute_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(orx_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_capi_library_exports_every_declared_symbol():
+    names = _declared_functions()
+    assert "orx_route_wave" in names and "orx_sssp_csr" in names and "orx_route_waves" in names
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(names) == bound, set(names) ^ bound
+    lib = _lib.load()
+    for n in names:
+        assert getattr(lib, n) is not None
+    assert lib.orx_version() >= 10000
