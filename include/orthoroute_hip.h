@@ -122,6 +122,7 @@ typedef struct orx_route_cfg {
                             /* above 2^18 ROI nodes)                         */
 #define ORX_F_SEED_PEN 4    /* CSR seeds pay their own node penalty (multi-  */
                             /* seed forms UP:2582-2583, CD:5589-5606)        */
+                            /* (bits 16 and up are reserved)                 */
 
 /* stats[]: 0 settled nodes (label-setting count up to the sink), 1 relaxed
  * arcs (their in-ROI out-degree), 2 queue items processed, 3 near rounds,
@@ -162,11 +163,14 @@ int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* 
  * the nets d_nets[h_wave_ptr[i] .. h_wave_ptr[i+1]) with workspace offsets
  * d_ws_off[same range], routed as by orx_route_wave, waves in order on one
  * stream (the per-net loop of _route_all, UP:7949-8430, batched).
- * h_wave_ptr[n_waves+1] and h_wave_max[n_waves] (largest ROI of each wave)
- * are HOST arrays. */
+ * h_wave_ptr[n_waves+1], h_wave_max[n_waves] (largest ROI of each wave)
+ * and h_wave_ws[n_waves] (the wave's nets own workspace nodes inside
+ * [0, h_wave_ws[i]); the library clears that range chip-wide before the
+ * wave when its largest ROI keeps search state in the workspace) are HOST
+ * arrays. */
 int orx_route_waves(const orx_lattice* lat, const orx_params* p, const orx_nets* nets,
                     const int32_t* d_nets, const int64_t* d_ws_off, int32_t n_waves,
-                    const int32_t* h_wave_ptr, const int64_t* h_wave_max,
+                    const int32_t* h_wave_ptr, const int64_t* h_wave_max, const int64_t* h_wave_ws,
                     const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream);
 
 /* Launch-sequence capture into a hipGraph (no reference counterpart: the

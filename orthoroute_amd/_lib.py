@@ -91,7 +91,7 @@ SIGNATURES = [
     ("orx_route_wave", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
                                  C.c_int32, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
     ("orx_route_waves", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
-                                  C.c_int32, P, P, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
+                                  C.c_int32, P, P, P, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
     ("orx_graph_begin", C.c_int, [C.POINTER(C.c_void_p)]),
     ("orx_graph_end", C.c_int, [P, C.POINTER(C.c_void_p)]),
     ("orx_graph_launch", C.c_int, [P, P]),

@@ -200,6 +200,9 @@ __device__ __forceinline__ unsigned long long dbits(double d) { return (unsigned
 // ---------------------------------------------------------------------------
 // Entries of the LDS ring that holds the first part of a round's near queue.
 constexpr uint32_t kRing = 2048;
+// Internal route flag (above the public ORX_F_* bits): the launch's global
+// workspace range was cleared by ws_fill_kernel just before it.
+constexpr int32_t kPrefilled = 1 << 16;
 
 __device__ __forceinline__ uint32_t ring_get(const uint32_t* lq, const uint32_t* gq, uint32_t i) {
     return i < kRing ? lq[i] : gq[i];
@@ -209,15 +212,20 @@ __device__ __forceinline__ void ring_put(uint32_t* lq, uint32_t* gq, uint32_t i,
     else gq[i] = v;
 }
 
+// skip_keys / skip_bitmaps: those arrays were cleared before the launch
+// (global workspace ranges filled chip-wide by ws_fill_kernel).
 template <int BLOCK>
 __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rdst, bool full, float delta,
                          const Heur& H, uint64_t* keys, uint32_t* bmn0, uint32_t* bmn1, uint32_t* bmf, uint32_t* qA,
-                         uint32_t* qB, uint32_t* lqA, uint32_t* lqB, uint32_t* fA, uint32_t* fB, Ctl* ctl) {
+                         uint32_t* qB, uint32_t* lqA, uint32_t* lqB, uint32_t* fA, uint32_t* fB, bool skip_keys,
+                         bool skip_bitmaps, Ctl* ctl) {
     const uint32_t tid = threadIdx.x;
     const uint32_t R = B.R;
     const uint32_t nwords = (R + 31) / 32;
-    for (uint32_t r = tid; r < R; r += BLOCK) keys[r] = ORX_KEY_INF;
-    for (uint32_t i = tid; i < nwords; i += BLOCK) { bmn0[i] = 0; bmn1[i] = 0; bmf[i] = 0; }
+    if (!skip_keys)
+        for (uint32_t r = tid; r < R; r += BLOCK) keys[r] = ORX_KEY_INF;
+    if (!skip_bitmaps)
+        for (uint32_t i = tid; i < nwords; i += BLOCK) { bmn0[i] = 0; bmn1[i] = 0; bmf[i] = 0; }
     if (tid == 0) {
         ctl->qcnt[0] = 1; ctl->qcnt[1] = 0; ctl->qcnt[2] = 0;
         ctl->fcnt = 0; ctl->fcnt2 = 0; ctl->err = 0;
@@ -487,6 +495,7 @@ struct SlotState {
     uint64_t* keys;
     uint32_t *bmn0, *bmn1, *bmf, *qA, *qB, *fA, *fB;
     uint32_t *lqA, *lqB;
+    bool keys_global, bm_global;   // kept in the global workspace
 };
 
 __device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t off, uint32_t R, uint32_t* lds,
@@ -498,6 +507,8 @@ __device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t
     S.fB = ws.queues + 3 * ws.ws_nodes + off;
     S.lqA = lds;
     S.lqB = lds + kRing;
+    S.keys_global = false;
+    S.bm_global = false;
     uint32_t* region = lds + 2 * kRing;
     const uint32_t words = (R + 31) / 32;
     const int64_t bm_bytes = 3ll * words * 4;
@@ -508,9 +519,11 @@ __device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t
         S.bmn0 = bm; S.bmn1 = bm + words; S.bmf = bm + 2 * words;
     } else {
         S.keys = ws.keys + off;
+        S.keys_global = true;
         if (bm_bytes <= (int64_t)region_bytes) {
             S.bmn0 = region; S.bmn1 = region + words; S.bmf = region + 2 * words;
         } else {
+            S.bm_global = true;
             S.bmn0 = ws.bitmaps + off / 32;
             S.bmn1 = ws.bitmaps + ws.ws_nodes / 32 + off / 32;
             S.bmf = ws.bitmaps + 2 * (ws.ws_nodes / 32) + off / 32;
@@ -549,15 +562,16 @@ __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, in
     } else {
         const bool full = (flags & ORX_F_FULL) != 0;
         Heur H = make_heur(g, P, nets.dst[k], !full);
-        box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA, S.fB,
-                        &ctl);
+        const bool pre = (flags & kPrefilled) != 0;
+        box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB,
+                        S.fA, S.fB, pre && S.keys_global, pre && S.bm_global, &ctl);
         if (H.on) {
             const uint64_t kd = key_load(S.keys + rdst);
             if (kd != ORX_KEY_INF && !(kdist(kd) < P.astar_gmax)) {
                 // distances this large may round below the A* margin: redo plainly
                 H.on = false;
-                box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA,
-                                S.fB, &ctl);
+                box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA,
+                                S.lqB, S.fA, S.fB, false, false, &ctl);
             }
         }
         if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
@@ -638,6 +652,25 @@ __global__ __launch_bounds__(BLOCK) void route_dag_kernel(Lat g, Params P, NetsD
     }
 }
 
+// Clears the first n_keys keys (all-ones) and the first n_words words of
+// each of the three bitmap arrays of a workspace, chip-wide and coalesced:
+// the search state of the largest ROIs of a wave, which one workgroup
+// would otherwise clear alone.
+__global__ __launch_bounds__(256) void ws_fill_kernel(orx_workspace ws, int64_t n_keys, int64_t n_words) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    ulonglong2* k2 = reinterpret_cast<ulonglong2*>(ws.keys);
+    const ulonglong2 inf2 = make_ulonglong2(ORX_KEY_INF, ORX_KEY_INF);
+    for (int64_t i = t; i < n_keys / 2; i += stride) k2[i] = inf2;
+    if (t == 0 && (n_keys & 1)) ws.keys[n_keys - 1] = ORX_KEY_INF;
+    const int64_t wn = ws.ws_nodes / 32;
+    for (int64_t i = t; i < n_words; i += stride) {
+        ws.bitmaps[i] = 0u;
+        ws.bitmaps[wn + i] = 0u;
+        ws.bitmaps[2 * wn + i] = 0u;
+    }
+}
+
 __global__ void zero_u32_kernel(uint32_t* p) {
     if (threadIdx.x == 0) p[0] = 0;
 }
@@ -665,7 +698,8 @@ __global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws
     }
     Heur H;
     H.on = false;
-    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA, S.fB, &ctl);
+    box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA,
+                    S.fB, false, false, &ctl);
     if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
     for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
         const uint64_t k = key_load(S.keys + r);
@@ -1243,21 +1277,53 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
 
 // One route_wave_kernel launch (K nets, one workgroup each) after the
 // argument checks shared by orx_route_wave and orx_route_waves.
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            return 1;
+    }
+    return cus;
+}
+
+// ws_extent > 0: the wave's nets own workspace nodes inside [0, ws_extent).
+// If the largest ROI keeps its keys in the global workspace, that range is
+// cleared chip-wide first and the searches skip their own clear.
 int launch_route_wave(const Lat& g, const Params& P, const NetsDev& nd, const int32_t* d_wave_nets,
-                      const int64_t* d_ws_off, int32_t K, int64_t max_nodes, const orx_workspace* ws,
-                      const orx_route_cfg* cfg, hipStream_t s) {
+                      const int64_t* d_ws_off, int32_t K, int64_t max_nodes, int64_t ws_extent,
+                      const orx_workspace* ws, const orx_route_cfg* cfg, hipStream_t s) {
     if (K <= 0) return ORX_OK;
     if (max_nodes < 1 || max_nodes >= (1ll << 24)) return orx_host::fail(ORX_E_ARG, "ROI size outside [1, 2^24)");
     const LdsPlan lp = lds_plan(max_nodes, cfg->lds_nodes);
     const size_t lds = lp.bytes;
     unsigned long long* st = (unsigned long long*)cfg->stats;
     int rc = ORX_OK;
+    int32_t flags = cfg->flags & ~kPrefilled;
+    if (ws_extent > 0) {
+        // whether a net keeps keys / bitmaps in the workspace grows with its
+        // ROI, so the largest ROI of the wave decides
+        const int64_t words = (max_nodes + 31) / 32;
+        const bool keys_global = !(max_nodes <= lp.lds_nodes && 8 * max_nodes + 12 * words <= lp.region);
+        const bool bm_global = 12 * words > lp.region;
+        if (keys_global) {
+            const int64_t ext = std::min<int64_t>(ws_extent, ws->ws_nodes);
+            const int64_t per = std::max<int64_t>(ext / 2, (ext + 31) / 32);
+            const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(device_cus() * 8, (per + 255) / 256));
+            hipLaunchKernelGGL(ws_fill_kernel, dim3(grid), dim3(256), 0, s, *ws, ext,
+                               bm_global ? (ext + 31) / 32 : (int64_t)0);
+            flags |= kPrefilled;
+        }
+    }
 #define ORX_LAUNCH_ROUTE(BS)                                                                              \
     do {                                                                                                  \
         rc = set_lds<route_wave_kernel<BS>>(lds);                                                         \
         if (rc) return rc;                                                                                \
         hipLaunchKernelGGL(route_wave_kernel<BS>, dim3(K), dim3(BS), lds, s, g, P, nd, d_wave_nets,       \
-                           d_ws_off, *ws, cfg->delta, lp.lds_nodes, lp.region, cfg->flags, st);           \
+                           d_ws_off, *ws, cfg->delta, lp.lds_nodes, lp.region, flags, st);                \
     } while (0)
     switch (cfg->block) {
     case 256: ORX_LAUNCH_ROUTE(256); break;
@@ -1330,30 +1396,35 @@ int orx_route_wave(const orx_lattice* lat, const orx_params* p, const orx_nets* 
     if (!p || !nets || !ws || !cfg || !d_wave_nets || !d_ws_off) return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (K <= 0) return ORX_OK;
     if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
-    return launch_route_wave(make_lat(lat), make_params(p), make_nets(nets), d_wave_nets, d_ws_off, K, max_nodes, ws,
-                             cfg, (hipStream_t)stream);
+    return launch_route_wave(make_lat(lat), make_params(p), make_nets(nets), d_wave_nets, d_ws_off, K, max_nodes, 0,
+                             ws, cfg, (hipStream_t)stream);
 }
 
 int orx_route_waves(const orx_lattice* lat, const orx_params* p, const orx_nets* nets, const int32_t* d_nets,
                     const int64_t* d_ws_off, int32_t n_waves, const int32_t* h_wave_ptr, const int64_t* h_wave_max,
-                    const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream) {
+                    const int64_t* h_wave_ws, const orx_workspace* ws, const orx_route_cfg* cfg,
+                    orx_stream_t stream) {
     int rc = check_lattice(lat);
     if (rc) return rc;
-    if (!p || !nets || !ws || !cfg || !h_wave_ptr || !h_wave_max) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (!p || !nets || !ws || !cfg || !h_wave_ptr || !h_wave_max || !h_wave_ws)
+        return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (n_waves <= 0) return ORX_OK;
     if (!d_nets || !d_ws_off) return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
     if (h_wave_ptr[0] != 0) return orx_host::fail(ORX_E_ARG, "wave_ptr[0] must be 0");
-    for (int32_t i = 0; i < n_waves; ++i)
+    for (int32_t i = 0; i < n_waves; ++i) {
         if (h_wave_ptr[i + 1] < h_wave_ptr[i] || h_wave_ptr[i + 1] > nets->n)
             return orx_host::fail(ORX_E_ARG, "wave_ptr must be non-decreasing and at most n");
+        if (h_wave_ws[i] < 0 || h_wave_ws[i] > ws->ws_nodes)
+            return orx_host::fail(ORX_E_ARG, "wave workspace extent outside [0, ws_nodes]");
+    }
     const Lat g = make_lat(lat);
     const Params P = make_params(p);
     const NetsDev nd = make_nets(nets);
     const hipStream_t s = (hipStream_t)stream;
     for (int32_t i = 0; i < n_waves; ++i) {
         const int32_t a = h_wave_ptr[i], K = h_wave_ptr[i + 1] - a;
-        rc = launch_route_wave(g, P, nd, d_nets + a, d_ws_off + a, K, h_wave_max[i], ws, cfg, s);
+        rc = launch_route_wave(g, P, nd, d_nets + a, d_ws_off + a, K, h_wave_max[i], h_wave_ws[i], ws, cfg, s);
         if (rc) return rc;
     }
     return ORX_OK;

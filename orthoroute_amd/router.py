@@ -316,14 +316,16 @@ def footprint_estimate(netlist: Netlist, cfg: RouterConfig | None = None) -> int
 def _flatten_waves(waves: list[np.ndarray], roi_nodes: np.ndarray):
     """Launch plan of orx_route_waves: net ids and 64-aligned workspace
     offsets (restarting at 0 in every wave), wave_ptr[n_waves+1],
-    wave_max[n_waves] (largest ROI of each wave) and the workspace size the
-    largest wave needs."""
+    wave_max[n_waves] (largest ROI of each wave), wave_ws[n_waves] (the
+    workspace nodes each wave occupies) and the workspace size the largest
+    wave needs."""
     nw = len(waves)
     sizes = np.fromiter((len(w) for w in waves), np.int64, nw)
     wave_ptr = np.zeros(nw + 1, np.int32)
     np.cumsum(sizes, out=wave_ptr[1:])
     if nw == 0:
-        return (np.zeros(0, np.int32), np.zeros(0, np.int64), wave_ptr, np.zeros(0, np.int64), 0)
+        return (np.zeros(0, np.int32), np.zeros(0, np.int64), wave_ptr, np.zeros(0, np.int64),
+                np.zeros(0, np.int64), 0)
     flat = np.concatenate(waves).astype(np.int32)
     R = roi_nodes[flat].astype(np.int64)
     r64 = (R + 63) // 64 * 64
@@ -331,8 +333,8 @@ def _flatten_waves(waves: list[np.ndarray], roi_nodes: np.ndarray):
     start = np.repeat(incl[wave_ptr[:-1]] - r64[wave_ptr[:-1]], sizes)
     off = (incl - r64 - start).astype(np.int64)
     wave_max = np.maximum.reduceat(R, wave_ptr[:-1]) if len(flat) else np.zeros(nw, np.int64)
-    totals = np.add.reduceat(r64, wave_ptr[:-1])
-    return flat, off, wave_ptr, wave_max.astype(np.int64), int(totals.max())
+    totals = np.add.reduceat(r64, wave_ptr[:-1]).astype(np.int64)
+    return flat, off, wave_ptr, wave_max.astype(np.int64), totals, int(totals.max())
 
 
 @dataclass
@@ -480,7 +482,7 @@ class DevicePathFinder:
             order = self.order_for(it)
         if waves is None:
             waves = self.plan(order, mode)
-        flat_nets, flat_off, wave_ptr, wave_max, need = _flatten_waves(waves, self.nets.roi_nodes)
+        flat_nets, flat_off, wave_ptr, wave_max, wave_ws, need = _flatten_waves(waves, self.nets.roi_nodes)
         s = self._s()
         ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(4))
         with torch.cuda.stream(stream):
@@ -495,7 +497,7 @@ class DevicePathFinder:
             exec_graph = None
             route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct), d_nets.data_ptr(),
                           d_off.data_ptr(), len(waves), wave_ptr.ctypes.data, wave_max.ctypes.data,
-                          C.byref(self.ws.struct), C.byref(rcfg))
+                          wave_ws.ctypes.data, C.byref(self.ws.struct), C.byref(rcfg))
             if graph and len(waves):
                 cs = C.c_void_p()
                 _lib.call("orx_graph_begin", C.byref(cs))

@@ -106,7 +106,7 @@ class CpuBuildPathFinder:
             "src": self.nl.src.astype(np.int32), "dst": self.nl.dst.astype(np.int32),
             "path_arena": self.arena, "path_off": self.path_off, "path_cap": self.path_cap,
             "path_len": self.path_len, "path_cost": self.path_cost, "status": self.status,
-            "sched": sched.astype(np.int32), "dep_ptr": dep_ptr.astype(np.int32),
+            "sched": sched.astype(np.int32), "wave": wave.astype(np.int32), "dep_ptr": dep_ptr.astype(np.int32),
             "dep_ids": dep_ids.astype(np.int32) if len(dep_ids) else np.zeros(1, np.int32),
             "ws_off": self.ws_off, "done": self.done,
         }

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -108,8 +109,29 @@ int main(int argc, char** argv) {
                       stats.data()};
     CHECK(orx_init());
     CHECK(orx_lattice_reprice(&lat, &P, nullptr));
-    const int64_t mode = M("mode");  // 0 = dependency-driven launch, 1 = one launch per net in sched order
-    if (mode == 0) {
+    // 0 = dependency-driven launch, 1 = one launch per net in sched order,
+    // 2 = the wave sequence through orx_route_waves (sched is ordered by wave)
+    const int64_t mode = M("mode");
+    if (mode == 2) {
+        auto wave = load<int32_t>("wave", n);
+        std::vector<int32_t> ptr{0};
+        std::vector<int64_t> wmax, wws, offs((size_t)n);
+        int64_t acc = 0, mx = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t k = sched[i];
+            if (i > 0 && wave[k] != wave[sched[i - 1]]) {
+                ptr.push_back(i); wmax.push_back(mx); wws.push_back(acc);
+                acc = 0; mx = 0;
+            }
+            const int64_t R = (int64_t)box[4 * (size_t)k + 2] * box[4 * (size_t)k + 3] * L;
+            offs[(size_t)i] = acc;
+            acc += (R + 63) / 64 * 64;
+            mx = std::max(mx, R);
+        }
+        ptr.push_back(n); wmax.push_back(mx); wws.push_back(acc);
+        CHECK(orx_route_waves(&lat, &P, &nets, sched.data(), offs.data(), (int32_t)wmax.size(), ptr.data(),
+                              wmax.data(), wws.data(), &ws, &cfg, nullptr));
+    } else if (mode == 0) {
         CHECK(orx_route_dag(&lat, &P, &nets, sched.data(), n, dep_ptr.data(), dep_ids.data(), done.data(),
                             (int32_t)M("epoch"), counter.data(), ws_off.data(), M("max_nodes"), &ws, &cfg,
                             (int32_t)M("max_groups"), nullptr));

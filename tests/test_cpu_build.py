@@ -21,12 +21,17 @@ def driver():
     return build_driver()
 
 
-@pytest.mark.parametrize("mode,groups,conc", [(1, 0, 1), (0, 3, 3)])
-def test_route_kernels_on_host_threads(driver, tmp_path, monkeypatch, mode, groups, conc):
+@pytest.mark.parametrize("mode,groups,conc,lds_nodes", [(1, 0, 1, 12288), (0, 3, 3, 12288), (1, 0, 1, 0),
+                                                        (2, 0, 3, 0), (2, 0, 2, 12288)])
+def test_route_kernels_on_host_threads(driver, tmp_path, monkeypatch, mode, groups, conc, lds_nodes):
+    """mode 1: one launch per net; 0: the dependency-driven launch; 2: the
+    wave sequence of orx_route_waves, whose waves get their global workspace
+    range cleared chip-wide before they run (lds_nodes=0 keeps every net's
+    keys there)."""
     from tests.cpu_build.harness import CpuBuildPathFinder
     monkeypatch.setenv("HIPCPU_CONCURRENT_WG", str(conc))
     nl = uneven(24, 20, 3, 10, seed=4)
-    cfg = RouterConfig(iterations=2)
+    cfg = RouterConfig(iterations=2, lds_nodes=lds_nodes)
     c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=mode, max_groups=groups)
     o = OraclePathFinder(nl, cfg)
     for _ in range(2):

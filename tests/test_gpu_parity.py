@@ -85,6 +85,19 @@ def test_block_sizes_bit_exact(device, block):
     _run_both(nl, cfg, 3, "batched", device)
 
 
+def test_largest_rois_bit_exact(device):
+    """The 16 largest nets of the 800x800x20 board (ROIs of 0.5-2.5 M nodes):
+    keys and bitmaps in the global workspace, the near queue spilling past
+    its LDS ring, the touched-node reset -- all against the oracle."""
+    from orthoroute_amd.netgen import uneven_800
+    nl = uneven_800()
+    big = np.argsort(-nl.roi_nodes(), kind="stable")[:16]
+    sub = nl.select(np.sort(big), "largest16")
+    assert sub.roi_nodes().min() > 400_000
+    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384)
+    _run_both(sub, cfg, 2, "batched", device)
+
+
 def test_backplane_subset_bit_exact(device):
     nl = backplane_400()
     sub = nl.subset(200)

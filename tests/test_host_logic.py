@@ -59,15 +59,16 @@ def test_waves_are_disjoint_and_keep_sequential_order():
 def test_flatten_waves_offsets_and_sizes():
     R = np.array([10, 64, 65, 1, 200, 7], np.int64)
     waves = [np.array([2, 0], np.int32), np.array([4], np.int32), np.array([1, 5, 3], np.int32)]
-    flat, off, ptr, wmax, need = _flatten_waves(waves, R)
+    flat, off, ptr, wmax, wws, need = _flatten_waves(waves, R)
     assert flat.tolist() == [2, 0, 4, 1, 5, 3]
     assert ptr.tolist() == [0, 2, 3, 6]
     assert off.tolist() == [0, 128, 0, 0, 64, 128]
     assert wmax.tolist() == [65, 200, 64]
+    assert wws.tolist() == [128 + 64, 256, 64 + 64 + 64]
     assert need == max(128 + 64, 256, 64 + 64 + 64)
     assert (off % 64 == 0).all()
     e = _flatten_waves([], R)
-    assert e[2].tolist() == [0] and e[4] == 0
+    assert e[2].tolist() == [0] and e[5] == 0
 
 
 def test_pres_schedule_is_exact_fp32_and_capped():
