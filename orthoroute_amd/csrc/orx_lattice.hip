@@ -51,6 +51,7 @@ struct Ctl {
     int32_t err;
     int32_t path_len;
     uint32_t rounds, phases, items;
+    uint32_t fhist[32];   // far-pile histogram of log2(1 + (f - T) / delta)
     unsigned long long settled, relaxed;
 };
 
@@ -337,8 +338,10 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
         }
         const uint32_t nf = ctl->fcnt;
         if (nf == 0) break;
-        // split pass 1: drop settled far entries, compact live ones, find min f
+        // split pass 1: drop settled far entries, compact live ones, find
+        // min f and a log2 histogram of how far above T the live ones are
         if (tid == 0) { ctl->fcnt2 = 0; ctl->minf = dbits((double)INFINITY); ctl->phases++; }
+        if (tid < 32) ctl->fhist[tid] = 0;
         __syncthreads();
         double lmin = INFINITY;
         for (uint32_t base = 0; base < nf; base += BLOCK) {
@@ -352,7 +355,14 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
                 roi_decode(B, v, lx, ly, z);
                 const double f = (double)kdist(key_load(keys + v)) + box_h(H, B, lx, ly, z);
                 if (f < T) bit_clear(bmf, v);
-                else { keep = true; lmin = fmin(lmin, f); }
+                else {
+                    keep = true;
+                    lmin = fmin(lmin, f);
+                    const double y = 1.0 + (f - T) / (double)delta;
+                    int b = (y < 2.0) ? 0 : ilogb(y);
+                    b = b > 31 ? 31 : b;
+                    atomicAdd(&ctl->fhist[b], 1u);
+                }
             }
             const int at = wave_append(keep, &ctl->fcnt2);
             if (keep) fB[at] = v;
@@ -363,7 +373,24 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
         const uint32_t nf2 = ctl->fcnt2;
         if (nf2 == 0) break;
         const double mn = __longlong_as_double((long long)ctl->minf);
+        // Next threshold: at least delta above the minimum, and wide enough
+        // that the next near phase gets about two workgroups' worth of far
+        // entries.  With costs spread far beyond delta (a congested board
+        // late in the negotiation) a fixed delta would run a phase per few
+        // nodes and rescan the whole far pile each time.  Any threshold
+        // above the minimum reaches the same fixpoint.
         double Tn = mn + (double)delta;
+        if (!isinf(delta)) {
+            const uint32_t target = nf2 < 2u * BLOCK ? nf2 : 2u * BLOCK;
+            uint32_t c = 0;
+            int b = 31;
+            for (int i = 0; i < 32; ++i) {
+                c += ctl->fhist[i];
+                if (c >= target) { b = i; break; }
+            }
+            const double edge = T + (double)delta * (ldexp(1.0, b + 1) - 1.0);
+            if (edge > Tn) Tn = edge;
+        }
         if (!(Tn > mn)) Tn = nextafter(mn, (double)INFINITY);
         if (tid == 0) ctl->fcnt = 0;
         __syncthreads();

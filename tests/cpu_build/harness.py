@@ -131,7 +131,7 @@ class CpuBuildPathFinder:
         return {"routed": int(stats[5]), "failed": int(stats[6]), "edge_over": int(out[0]),
                 "edge_over_count": int(out[1]), "node_over": int(out[2]), "node_over_count": int(out[3]),
                 "via_hops": int(out[4]), "barrels": int(out[5]), "settled": int(stats[0]), "relaxed": int(stats[1]),
-                "path_hops": int(stats[7])}
+                "path_hops": int(stats[7]), "rounds": int(stats[3]), "phases": int(stats[4])}
 
     def paths(self):
         return [self.arena[o:o + l].copy() for o, l in zip(self.path_off, self.path_len)]

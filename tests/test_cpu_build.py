@@ -45,6 +45,28 @@ def test_route_kernels_on_host_threads(driver, tmp_path, monkeypatch, mode, grou
         assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
 
 
+def test_congested_board_many_threshold_phases(driver, tmp_path, monkeypatch):
+    """A small board packed with nets, three iterations: history and present
+    costs spread the distances far beyond delta, so the searches run many
+    threshold phases with the adaptive next threshold -- still bit-exact."""
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "2")
+    nl = uneven(16, 12, 3, 24, seed=8)
+    cfg = RouterConfig(iterations=3, delta=1.0, lds_nodes=0)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=2)
+    o = OraclePathFinder(nl, cfg)
+    phases = 0
+    for _ in range(3):
+        d = c.run_iteration(timeout=600)
+        od = o.run_iteration()
+        for k in ("routed", "edge_over", "node_over", "via_hops", "barrels", "settled", "relaxed"):
+            assert d[k] == od[k], k
+        for a, b in zip(c.paths(), o.paths()):
+            assert np.array_equal(a, b)
+        phases += d["phases"]
+    assert phases > 100
+
+
 def _bfs_layer_max(spec, box, src):
     """Largest hop layer of a breadth-first search from src inside the box."""
     x0, y0, w, h = box
