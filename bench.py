@@ -38,6 +38,10 @@ from pathlib import Path
 
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
+# The serving leg keeps up to 8 jobs in flight, one HIP stream each; with
+# fewer hardware queues than streams, jobs sharing a queue run in lockstep.
+# Must be set before the HIP runtime initialises.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"

@@ -94,6 +94,36 @@ class LatticeSpec:
         base[nb < 0] = np.inf
         return base.reshape(-1)
 
+    def base_costs_device(self, device, planar=PLANAR_COST, via=VIA_COST):
+        """The same [4N] array as base_costs, built on the device (no host
+        array, no host-to-device copy) with torch ops on the current stream."""
+        import torch
+        dev = torch.device(device)
+        u = torch.arange(self.N, dtype=torch.int64, device=dev)
+        z = u // self.XY
+        rem = u - z * self.XY
+        y = rem // self.X
+        x = rem - y * self.X
+        horiz = ((self.hmask >> z) & 1).bool()
+        inf = torch.tensor(float("inf"), dtype=torch.float32, device=dev)
+        pv = torch.tensor(float(np.float32(planar)), dtype=torch.float32, device=dev)
+        vv = torch.tensor(float(np.float32(via)), dtype=torch.float32, device=dev)
+        base = torch.empty((self.N, 4), dtype=torch.float32, device=dev)
+        base[:, 0] = torch.where(z > 0, vv, inf)
+        base[:, 3] = torch.where(z < self.L - 1, vv, inf)
+        base[:, 1] = torch.where(torch.where(horiz, x > 0, y > 0), pv, inf)
+        base[:, 2] = torch.where(torch.where(horiz, x < self.X - 1, y < self.Y - 1), pv, inf)
+        return base.reshape(-1)
+
+    def base_cost_minima(self, planar=PLANAR_COST, via=VIA_COST) -> np.ndarray:
+        """A tiny array whose planar / via minima equal those of base_costs
+        (for the A* bounds without building the whole array)."""
+        b = np.full((1, 4), np.inf, np.float32)
+        b[0, 1] = planar
+        if self.L > 1:
+            b[0, 3] = via
+        return b.reshape(-1)
+
     def random_edge_costs(self, seed: int, lo=1.0, hi=4.0, hot_frac=0.05, hot_mult=10.0) -> np.ndarray:
         """[4N] costs uniform in [lo, hi) per physical edge (shared by both
         arcs) with a seeded fraction multiplied by hot_mult (the 1000x1000x12

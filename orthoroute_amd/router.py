@@ -160,11 +160,14 @@ class LatticeState:
         self.spec = spec
         self.device = torch.device(device)
         N = spec.N
-        b = spec.base_costs() if base is None else np.asarray(base, np.float32)
-        if b.shape != (4 * N,):
-            raise ValueError("base must have 4*N entries")
         dev = self.device
-        self.base = torch.from_numpy(b).to(dev)
+        if base is None:
+            self.base = spec.base_costs_device(dev)
+        else:
+            b = np.asarray(base, np.float32)
+            if b.shape != (4 * N,):
+                raise ValueError("base must have 4*N entries")
+            self.base = torch.from_numpy(b).to(dev)
         self.hist = torch.zeros(4 * N, dtype=torch.float32, device=dev)
         self.occ = torch.zeros(4 * N, dtype=torch.int32, device=dev)
         self.wcost = torch.empty(4 * N, dtype=torch.float32, device=dev)
@@ -364,21 +367,29 @@ class DevicePathFinder:
         # every torch op and kernel of this job runs on one stream, so jobs on
         # different streams overlap on the device and never race
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
-        if base is None:
-            base = self.spec.base_costs()
+        n = netlist.n
         with torch.cuda.stream(self.stream):
             self.lat = LatticeState(self.spec, base, device)
             self.nets = NetTables(netlist, device)
             self.ws = Workspace(device)
             self._route_stats = torch.zeros(8, dtype=torch.int64, device=self.device)
             self._iter_out = torch.zeros(8, dtype=torch.int64, device=self.device)
-            self._over = torch.zeros(max(1, netlist.n), dtype=torch.int64, device=self.device)
-            self._hops = torch.zeros(max(1, netlist.n), dtype=torch.int32, device=self.device)
+            self._over = torch.zeros(max(1, n), dtype=torch.int64, device=self.device)
+            self._hops = torch.zeros(max(1, n), dtype=torch.int32, device=self.device)
+            self._plan_nets = torch.empty(max(1, n), dtype=torch.int32, device=self.device)
+            self._plan_off = torch.empty(max(1, n), dtype=torch.int64, device=self.device)
+        # pinned host staging: the launch path copies asynchronously on this
+        # job's stream and never waits for other streams' work
         self._iter_out_h = torch.zeros(8, dtype=torch.int64, pin_memory=True)
         self._route_stats_h = torch.zeros(8, dtype=torch.int64, pin_memory=True)
+        self._over_h = torch.zeros(max(1, n), dtype=torch.int64, pin_memory=True)
+        self._hops_h = torch.zeros(max(1, n), dtype=torch.int32, pin_memory=True)
+        self._plan_nets_h = torch.empty(max(1, n), dtype=torch.int32, pin_memory=True)
+        self._plan_off_h = torch.empty(max(1, n), dtype=torch.int64, pin_memory=True)
+        self.stream.synchronize()
         self.distance = net_distance(self.spec, netlist.src, netlist.dst)
         self.schedule = self.cfg.schedule(self.spec.L)
-        self.astar = self.cfg.astar_bounds(base)
+        self.astar = self.cfg.astar_bounds(self.spec.base_cost_minima() if base is None else base)
         self.iteration = 0
         self.max_groups = 0
         self.history: list[IterationStats] = []
@@ -431,8 +442,11 @@ class DevicePathFinder:
             with torch.cuda.stream(self.stream):
                 _lib.call("orx_net_overuse", C.byref(self.lat.struct), C.byref(self.nets.struct),
                           self.cfg.edge_cap, self._over.data_ptr(), self._hops.data_ptr(), self._s())
-                over = self._over[:n].cpu().numpy()
-                hops = self._hops[:n].cpu().numpy()
+                self._over_h[:n].copy_(self._over[:n], non_blocking=True)
+                self._hops_h[:n].copy_(self._hops[:n], non_blocking=True)
+            self.stream.synchronize()
+            over = self._over_h[:n].numpy().copy()
+            hops = self._hops_h[:n].numpy().copy()
         else:
             over = np.zeros(n, np.int64)
             hops = np.zeros(n, np.int32)
@@ -485,10 +499,18 @@ class DevicePathFinder:
         flat_nets, flat_off, wave_ptr, wave_max, wave_ws, need = _flatten_waves(waves, self.nets.roi_nodes)
         s = self._s()
         ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(4))
+        m = len(flat_nets)
+        if m > len(self._plan_nets_h):
+            raise ValueError("a launch plan lists more nets than the job has")
+        self._plan_nets_h.numpy()[:m] = flat_nets
+        self._plan_off_h.numpy()[:m] = flat_off
         with torch.cuda.stream(stream):
             self.ws.ensure(need)
-            d_nets = torch.from_numpy(flat_nets).to(self.device)
-            d_off = torch.from_numpy(flat_off).to(self.device)
+            d_nets = self._plan_nets[:max(m, 1)]
+            d_off = self._plan_off[:max(m, 1)]
+            if m:
+                d_nets[:m].copy_(self._plan_nets_h[:m], non_blocking=True)
+                d_off[:m].copy_(self._plan_off_h[:m], non_blocking=True)
             self._route_stats.zero_()
             ev[0].record(stream)
             self.lat.reprice(params, stream)

@@ -21,6 +21,13 @@ or its iteration limit.
 
 Reported per job: arrival, admission and finish times, iterations, final
 overuse, result digest; over the stream: throughput and p50/p99 latency.
+
+Start the process with at least as many HIP hardware queues as slots
+(GPU_MAX_HW_QUEUES, read when the HIP runtime starts; HIP's default is 4):
+streams that share a hardware queue execute in submission order, which ties
+a short job's iterations to a long job's.  Each job's launch path copies
+through pinned buffers on its own stream only, so no job waits on another
+job's work from the host side.
 This replaces nothing in the reference (its engine routes one board per
 process, UP:6163-7534); the PathFinder inside each job is the device path.
 """

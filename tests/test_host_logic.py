@@ -114,3 +114,20 @@ def test_capi_library_exports_every_declared_symbol():
     for n in names:
         assert getattr(lib, n) is not None
     assert lib.orx_version() >= 10000
+
+
+@pytest.mark.parametrize("dims", [(7, 5, 4), (16, 9, 1), (3, 3, 6)])
+def test_device_built_base_costs_equal_host_array(dims):
+    """LatticeState builds the default base costs with torch ops on the
+    device; the array must equal the host construction (run here on CPU)."""
+    from orthoroute_amd.lattice import LatticeSpec
+    spec = LatticeSpec(*dims)
+    got = spec.base_costs_device("cpu").numpy()
+    want = spec.base_costs()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    m = spec.base_cost_minima().reshape(-1, 4)
+    b = want.reshape(-1, 4)
+    assert m[:, 1:3][np.isfinite(m[:, 1:3])].min() == b[:, 1:3][np.isfinite(b[:, 1:3])].min()
+    via = b[:, [0, 3]][np.isfinite(b[:, [0, 3]])]
+    mv = m[:, [0, 3]][np.isfinite(m[:, [0, 3]])]
+    assert (via.size == 0 and mv.size == 0) or mv.min() == via.min()
