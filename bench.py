@@ -41,7 +41,12 @@ sys.path.insert(0, str(REPO))
 # The serving leg keeps up to 8 jobs in flight, one HIP stream each; with
 # fewer hardware queues than streams, jobs sharing a queue run in lockstep.
 # Must be set before the HIP runtime initialises.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+try:
+    _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _queues = 0
+if _queues < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"

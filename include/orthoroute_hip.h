@@ -210,6 +210,14 @@ int64_t orx_plan_iteration(const int32_t* box, const int32_t* order, int32_t n, 
                            int32_t Y, const int64_t* roi_nodes, int32_t* wave, int32_t* sched,
                            int32_t* dep_ptr, int32_t* dep_ids, int64_t dep_cap);
 
+/* Host planner, waves only: the same wave[n] as orx_plan_iteration and the
+ * schedule sched[n] (by wave, larger ROI first, then routing order),
+ * without dependency lists, on a tiled grid (8x8-cell tiles) so large boxes
+ * cost their tile count plus their border cells.  Returns the number of
+ * waves, or < 0 on bad input.  No device work. */
+int64_t orx_plan_waves(const int32_t* box, const int32_t* order, int32_t n, int32_t X, int32_t Y,
+                       const int64_t* roi_nodes, int32_t* wave, int32_t* sched);
+
 /* Per-net overuse of its committed path, summed over the forward arcs
  * (d_over[n]) and its hop count (d_hops[n]): the congestion input of
  * _order_nets_by_difficulty (UP:7840-7901). */

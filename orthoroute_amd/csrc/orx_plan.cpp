@@ -76,3 +76,80 @@ extern "C" int64_t orx_plan_iteration(const int32_t* box, const int32_t* order, 
     (void)nw;
     return total;
 }
+
+// Waves only (no dependency lists), for the per-iteration launch plan.
+// wave[k] = 1 + the largest wave of any earlier net (in `order`) whose box
+// overlaps net k's box -- the same waves as orx_plan_iteration -- computed on
+// a two-level grid: 8x8-cell tiles keep the largest wave of boxes covering
+// the whole tile (tfull) and the largest wave anywhere in the tile (tmax),
+// so a box visits its fully covered tiles once each and only the cells of
+// the tiles its border crosses.  A cell's wave is max(cell, tfull of its
+// tile).  Every box's wave exceeds every value inside it, so an update only
+// ever raises values.
+extern "C" int64_t orx_plan_waves(const int32_t* box, const int32_t* order, int32_t n, int32_t X, int32_t Y,
+                                  const int64_t* roi_nodes, int32_t* wave, int32_t* sched) {
+    if (n < 0 || X <= 0 || Y <= 0) return -1;
+    constexpr int32_t T = 8;
+    const int32_t TX = (X + T - 1) / T, TY = (Y + T - 1) / T;
+    std::vector<int32_t> cell((size_t)X * Y, -1), tfull((size_t)TX * TY, -1), tmax((size_t)TX * TY, -1);
+    int32_t nw = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t k = order[i];
+        const int32_t* b = box + 4 * (int64_t)k;
+        const int32_t x0 = b[0], y0 = b[1], x1 = b[0] + b[2], y1 = b[1] + b[3];
+        if (x0 < 0 || y0 < 0 || b[2] < 1 || b[3] < 1 || x1 > X || y1 > Y) return -2;
+        const int32_t tx0 = x0 / T, tx1 = (x1 - 1) / T, ty0 = y0 / T, ty1 = (y1 - 1) / T;
+        int32_t m = -1;
+        for (int32_t ty = ty0; ty <= ty1; ++ty) {
+            const int32_t cy0 = std::max(y0, ty * T), cy1 = std::min(y1, ty * T + T);
+            const bool full_y = cy0 == ty * T && cy1 == std::min(Y, ty * T + T) && cy1 - cy0 == std::min(T, Y - ty * T);
+            for (int32_t tx = tx0; tx <= tx1; ++tx) {
+                const size_t t = (size_t)ty * TX + tx;
+                const int32_t cx0 = std::max(x0, tx * T), cx1 = std::min(x1, tx * T + T);
+                const bool full = full_y && cx0 == tx * T && cx1 == std::min(X, tx * T + T);
+                if (full) {
+                    m = std::max(m, tmax[t]);
+                } else if (tmax[t] > m) {
+                    int32_t cm = tfull[t];
+                    for (int32_t y = cy0; y < cy1; ++y) {
+                        const int32_t* row = cell.data() + (size_t)y * X;
+                        for (int32_t x = cx0; x < cx1; ++x) cm = std::max(cm, row[x]);
+                    }
+                    m = std::max(m, cm);
+                }
+            }
+        }
+        const int32_t wk = m + 1;
+        wave[k] = wk;
+        nw = std::max(nw, wk + 1);
+        for (int32_t ty = ty0; ty <= ty1; ++ty) {
+            const int32_t cy0 = std::max(y0, ty * T), cy1 = std::min(y1, ty * T + T);
+            const bool full_y = cy0 == ty * T && cy1 - cy0 == std::min(T, Y - ty * T);
+            for (int32_t tx = tx0; tx <= tx1; ++tx) {
+                const size_t t = (size_t)ty * TX + tx;
+                const int32_t cx0 = std::max(x0, tx * T), cx1 = std::min(x1, tx * T + T);
+                const bool full = full_y && cx0 == tx * T && cx1 - cx0 == std::min(T, X - tx * T);
+                if (full) {
+                    tfull[t] = wk;
+                } else {
+                    for (int32_t y = cy0; y < cy1; ++y) {
+                        int32_t* row = cell.data() + (size_t)y * X;
+                        for (int32_t x = cx0; x < cx1; ++x) row[x] = wk;
+                    }
+                }
+                tmax[t] = std::max(tmax[t], wk);
+            }
+        }
+    }
+    // schedule: by wave, larger ROI first, then routing order (counting sort
+    // on the wave, a stable sort inside each wave)
+    std::vector<int32_t> start((size_t)nw + 1, 0);
+    for (int32_t i = 0; i < n; ++i) start[(size_t)wave[order[i]] + 1]++;
+    for (int32_t w = 0; w < nw; ++w) start[(size_t)w + 1] += start[(size_t)w];
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int32_t i = 0; i < n; ++i) sched[fill[(size_t)wave[order[i]]]++] = order[i];
+    for (int32_t w = 0; w < nw; ++w)
+        std::stable_sort(sched + start[(size_t)w], sched + start[(size_t)w + 1],
+                         [&](int32_t a, int32_t c) { return roi_nodes[a] > roi_nodes[c]; });
+    return nw;
+}

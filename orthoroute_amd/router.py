@@ -394,6 +394,7 @@ class DevicePathFinder:
         self.max_groups = 0
         self.history: list[IterationStats] = []
         self._pending: _Pending | None = None
+        self._staged_for = -1     # iteration whose order input is staged in _over_h / _hops_h
 
     # -- helpers -----------------------------------------------------------
     def _s(self) -> int:
@@ -428,6 +429,7 @@ class DevicePathFinder:
                 for n in names:
                     getattr(obj, n).copy_(snap[(owner, n)])
         self.iteration = snap["iteration"]
+        self._staged_for = -1
         del self.history[snap["history"]:]
 
     def route_cfg(self, flags: int = 0) -> _lib.OrxRouteCfg:
@@ -435,16 +437,25 @@ class DevicePathFinder:
         fl = flags | (_lib.ORX_F_STATS if c.collect_stats else 0)
         return _lib.OrxRouteCfg(float(c.delta), int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr())
 
+    def _stage_net_overuse(self) -> None:
+        """Enqueue the per-net overuse / hop counts of the committed paths and
+        their copy into pinned host memory on this job's stream."""
+        n = self.netlist.n
+        with torch.cuda.stream(self.stream):
+            _lib.call("orx_net_overuse", C.byref(self.lat.struct), C.byref(self.nets.struct),
+                      self.cfg.edge_cap, self._over.data_ptr(), self._hops.data_ptr(), self._s())
+            self._over_h[:n].copy_(self._over[:n], non_blocking=True)
+            self._hops_h[:n].copy_(self._hops[:n], non_blocking=True)
+
     def order_for(self, iteration: int) -> np.ndarray:
-        """_order_nets_by_difficulty from the device occupancy (one readback)."""
+        """_order_nets_by_difficulty from the device occupancy.  The per-net
+        overuse of the previous iteration is normally staged by that
+        iteration's launch already (no extra round trip here)."""
         n = self.netlist.n
         if iteration > 1 and n:
-            with torch.cuda.stream(self.stream):
-                _lib.call("orx_net_overuse", C.byref(self.lat.struct), C.byref(self.nets.struct),
-                          self.cfg.edge_cap, self._over.data_ptr(), self._hops.data_ptr(), self._s())
-                self._over_h[:n].copy_(self._over[:n], non_blocking=True)
-                self._hops_h[:n].copy_(self._hops[:n], non_blocking=True)
-            self.stream.synchronize()
+            if self._staged_for != iteration:
+                self._stage_net_overuse()
+                self.stream.synchronize()
             over = self._over_h[:n].numpy().copy()
             hops = self._hops_h[:n].numpy().copy()
         else:
@@ -538,6 +549,8 @@ class DevicePathFinder:
             ev[3].record(stream)
             self._iter_out_h.copy_(self._iter_out, non_blocking=True)
             self._route_stats_h.copy_(self._route_stats, non_blocking=True)
+            if self.netlist.n and it < len(self.schedule):
+                self._stage_net_overuse()     # input of the next iteration's order
             done = torch.cuda.Event()
             done.record(stream)
         self._pending = _Pending(it, pres, len(waves), t0, ev, done, exec_graph, (d_nets, d_off),
@@ -568,6 +581,7 @@ class DevicePathFinder:
             route_ms=evr0.elapsed_time(evr1), extra={"graph": p.graph is not None, "launched_nets": p.nets})
         self._pending = None
         self.iteration = p.it
+        self._staged_for = p.it + 1 if self.netlist.n and p.it < len(self.schedule) else -1
         self.history.append(st)
         return st
 

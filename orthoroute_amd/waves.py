@@ -128,9 +128,7 @@ def plan_iteration(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes
 def plan_waves(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes: np.ndarray) -> list[np.ndarray]:
     """The disjoint-ROI waves of one iteration from the native planner: the
     same waves and in-wave order as group_waves(color_waves(...)), computed
-    in C++ (no dependency lists)."""
-    import ctypes as C
-
+    in C++ on a tiled grid (orx_plan_waves)."""
     from . import _lib
     lib = _lib.load()
     n = len(order)
@@ -141,13 +139,10 @@ def plan_waves(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes: np
     rn = np.ascontiguousarray(roi_nodes, np.int64)
     wave = np.empty(n, np.int32)
     sched = np.empty(n, np.int32)
-    dep_ptr = np.empty(n + 1, np.int32)
-    dep_ids = np.empty(1, np.int32)
-    total = lib.orx_plan_iteration(box.ctypes.data, order.ctypes.data, n, X, Y, rn.ctypes.data,
-                                   wave.ctypes.data, sched.ctypes.data, dep_ptr.ctypes.data,
-                                   dep_ids.ctypes.data, 0)
-    if total < 0:
-        raise ValueError(f"orx_plan_iteration rejected its input ({total})")
+    nw = lib.orx_plan_waves(box.ctypes.data, order.ctypes.data, n, X, Y, rn.ctypes.data, wave.ctypes.data,
+                            sched.ctypes.data)
+    if nw < 0:
+        raise ValueError(f"orx_plan_waves rejected its input ({nw})")
     ws = wave[sched]
     cuts = np.flatnonzero(np.diff(ws)) + 1
     return np.split(sched, cuts)

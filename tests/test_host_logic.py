@@ -36,6 +36,20 @@ def test_native_waves_equal_python_colouring(k):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_tiled_planner_equals_cell_planner(seed):
+    """orx_plan_waves (tiled) gives exactly the waves of orx_plan_iteration
+    (per cell), on boards whose sides are not multiples of the tile."""
+    from orthoroute_amd.waves import plan_iteration
+    nl = uneven(203, 157, 4, 900, seed=seed)
+    order = np.random.default_rng(seed).permutation(nl.n).astype(np.int32)
+    R = nl.roi_nodes()
+    wave, sched, _, _ = plan_iteration(nl.box, order, nl.spec.X, nl.spec.Y, R)
+    got = plan_waves(nl.box, order, nl.spec.X, nl.spec.Y, R)
+    assert np.array_equal(np.concatenate(got), sched)
+    assert sum(len(w) for w in got) == nl.n and len(got) == int(wave.max()) + 1
+
+
 def test_waves_are_disjoint_and_keep_sequential_order():
     nl = uneven(120, 120, 4, 400, seed=11)
     order = np.random.default_rng(3).permutation(nl.n).astype(np.int32)
