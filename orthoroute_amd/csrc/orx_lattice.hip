@@ -455,28 +455,88 @@ __device__ void box_stats(const Lat& g, const Box& B, const Heur& H, const uint6
     }
 }
 
-// Parent walk from the sink by key slots (single lane).  Writes len global
-// ids, source first.  Returns len or a negative status.
-__device__ int32_t box_backtrace(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rsrc, uint32_t rdst,
-                                 int32_t* path, int32_t cap) {
+
+// Octahedron of radius 3 around a node: the 63 offsets (dx, dy, dz) with
+// |dx| + |dy| + |dz| <= 3 (every node three parent moves can reach), and
+// the lane of each offset, indexed (dx+3)*49 + (dy+3)*7 + (dz+3) (-1 = none).
+__device__ __constant__ int8_t kStencilDx[63] = {
+    0, 0, -1, 0, 1, 0, 0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0, 0, -1,
+    0, 1, -2, -1, 0, 1, 2, -3, -2, -1, 0, 1, 2, 3, -2, -1, 0, 1, 2, -1, 0,
+    1, 0, 0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0, 0, -1, 0, 1, 0, 0,
+};
+__device__ __constant__ int8_t kStencilDy[63] = {
+    0, -1, 0, 0, 0, 1, -2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2, -3, -2,
+    -2, -2, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2,
+    2, 3, -2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2, -1, 0, 0, 0, 1, 0,
+};
+__device__ __constant__ int8_t kStencilDz[63] = {
+    -3, -2, -2, -2, -2, -2, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 0,
+    0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+    0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 3,
+};
+__device__ __constant__ int8_t kStencilLane[343] = {
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 28, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 23, -1, -1, -1, -1, -1, 10, 29, 48, -1, -1, -1, -1, -1, 35, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 20, -1, -1, -1, -1, -1, 7, 24, 45, -1, -1, -1, 2, 11, 30, 49, 58, -1, -1, -1, 15, 36, 53, -1, -1, -1, -1, -1, 40, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+    -1, -1, -1, 19, -1, -1, -1, -1, -1, 6, 21, 44, -1, -1, -1, 1, 8, 25, 46, 57, -1, 0, 3, 12, 31, 50, 59, 62, -1, 5, 16, 37, 54, 61, -1, -1, -1, 18, 41, 56, -1, -1, -1, -1, -1, 43, -1, -1, -1,
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 22, -1, -1, -1, -1, -1, 9, 26, 47, -1, -1, -1, 4, 13, 32, 51, 60, -1, -1, -1, 17, 38, 55, -1, -1, -1, -1, -1, 42, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 27, -1, -1, -1, -1, -1, 14, 33, 52, -1, -1, -1, -1, -1, 39, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 34, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+};
+
+// Parent walk by one wavefront.  The lanes load the keys of the 63 nodes
+// within three moves of the current node at once; then every lane follows
+// up to four parent links through shuffles (the nodes it looks up are at
+// most three moves away; the same walk on every lane, so control flow stays
+// uniform).  One dependent memory round trip per four hops instead of one
+// per hop.  Lane 0 writes the path, root first.
+// Call with all 64 lanes of one wave; every lane returns the same value.
+__device__ int32_t box_backtrace_wave(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rsrc,
+                                      uint32_t rdst, int32_t* path, int32_t cap) {
+    const uint32_t lane = lane_id();
     const uint64_t kd = key_load(keys + rdst);
     if (kd == ORX_KEY_INF) return ORX_NO_PATH;
     const uint32_t hops = khops(kd);
     if ((int64_t)hops + 1 > cap) return ORX_E_OVERFLOW;
-    uint32_t v = rdst;
-    for (int32_t i = (int32_t)hops; i >= 0; --i) {
-        uint32_t lx, ly, z;
-        roi_decode(B, v, lx, ly, z);
-        path[i] = (int32_t)roi_global(g, B, lx, ly, z);
-        const uint32_t sl = kslot(key_load(keys + v));
-        if (i == 0) return (sl == ORX_ROOT_SLOT && v == rsrc) ? (int32_t)hops + 1 : ORX_E_CYCLE;
-        if (sl > 3) return ORX_E_CYCLE;
-        const bool horiz = (g.hmask >> z) & 1u;
-        const uint32_t p = roi_nbr(B, horiz, v, lx, ly, z, (int)sl);
-        if (p == UINT32_MAX) return ORX_E_CYCLE;
-        v = p;
+    const int32_t sdx = lane < 63 ? kStencilDx[lane] : 0;
+    const int32_t sdy = lane < 63 ? kStencilDy[lane] : 0;
+    const int32_t sdz = lane < 63 ? kStencilDz[lane] : 0;
+    uint32_t ux, uy, uz;
+    roi_decode(B, rdst, ux, uy, uz);
+    int32_t cx = (int32_t)ux, cy = (int32_t)uy, cz = (int32_t)uz;
+    int32_t i = (int32_t)hops;
+    for (;;) {
+        const int32_t nx = cx + sdx, ny = cy + sdy, nz = cz + sdz;
+        uint64_t mine = ORX_KEY_INF;
+        if (lane < 63 && nx >= 0 && nx < B.w && ny >= 0 && ny < B.h && nz >= 0 && nz < B.L)
+            mine = key_load(keys + ((uint32_t)nz * (uint32_t)B.h + (uint32_t)ny) * (uint32_t)B.w + (uint32_t)nx);
+        int32_t ox = 0, oy = 0, oz = 0;
+        for (int step = 0; step < 4; ++step) {
+            const int li = kStencilLane[(ox + 3) * 49 + (oy + 3) * 7 + (oz + 3)];
+            const uint64_t k = __shfl(mine, li);
+            const int32_t vx = cx + ox, vy = cy + oy, vz = cz + oz;
+            const uint32_t v = ((uint32_t)vz * (uint32_t)B.h + (uint32_t)vy) * (uint32_t)B.w + (uint32_t)vx;
+            if (lane == 0) path[i] = (int32_t)roi_global(g, B, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
+            const uint32_t sl = kslot(k);
+            if (i == 0) return (sl == ORX_ROOT_SLOT && v == rsrc) ? (int32_t)hops + 1 : ORX_E_CYCLE;
+            if (sl > 3) return ORX_E_CYCLE;
+            int32_t mx = 0, my = 0, mz = 0;
+            if (sl == 0) {
+                mz = -1;
+            } else if (sl == 3) {
+                mz = 1;
+            } else {
+                const int32_t d = sl == 1 ? -1 : 1;
+                if ((g.hmask >> vz) & 1u) mx = d;
+                else my = d;
+            }
+            const int32_t px = vx + mx, py = vy + my, pz = vz + mz;
+            if (px < 0 || px >= B.w || py < 0 || py >= B.h || pz < 0 || pz >= B.L) return ORX_E_CYCLE;
+            ox += mx; oy += my; oz += mz;
+            --i;
+        }
+        cx += ox; cy += oy; cz += oz;
     }
-    return ORX_E_CYCLE;
 }
 
 __device__ __forceinline__ Heur make_heur(const Lat& g, const Params& P, int64_t dst, bool allowed) {
@@ -603,12 +663,14 @@ __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, in
         }
         if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
         // 3. backtrace
-        if (threadIdx.x == 0) {
-            const int32_t l = box_backtrace(g, B, S.keys, rsrc, rdst, path, cap);
-            ctl.path_len = l;
-            nets.status[k] = l > 0 ? ORX_OK : l;
-            nets.path_len[k] = l > 0 ? l : 0;
-            nets.path_cost[k] = l > 0 ? kdist(key_load(S.keys + rdst)) : INFINITY;
+        if (threadIdx.x < 64) {
+            const int32_t l = box_backtrace_wave(g, B, S.keys, rsrc, rdst, path, cap);
+            if (threadIdx.x == 0) {
+                ctl.path_len = l;
+                nets.status[k] = l > 0 ? ORX_OK : l;
+                nets.path_len[k] = l > 0 ? l : 0;
+                nets.path_cost[k] = l > 0 ? kdist(key_load(S.keys + rdst)) : INFINITY;
+            }
         }
         __syncthreads();
         len = ctl.path_len;
@@ -1159,9 +1221,10 @@ int check_lattice(const orx_lattice* l) {
     return ORX_OK;
 }
 
-// Largest dynamic LDS a launch may ask for (160 KiB per CU, less the static
-// part of the kernels).
-constexpr size_t kMaxDynLds = 160 * 1024 - 256;
+// Largest dynamic LDS a launch may ask for: 160 KiB per CU less a reserve
+// for the kernels' static LDS (the search control block, a few hundred
+// bytes).
+constexpr size_t kMaxDynLds = 160 * 1024 - 1024;
 
 // Dynamic LDS of a search launch whose largest ROI has max_nodes nodes: the
 // two near-queue rings plus a region big enough for the keys and bitmaps of

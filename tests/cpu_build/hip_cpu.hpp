@@ -31,6 +31,7 @@
 
 #define __global__
 #define __device__
+#define __constant__
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(x)
