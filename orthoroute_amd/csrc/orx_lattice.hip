@@ -1314,7 +1314,11 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
         e = hipGetLastError();
     }
     const int block = cfg->block;
-    const int grid = std::max(1, cus * std::max(1, 512 / std::max(256, block)));
+    // Half a workgroup per CU: a step is short (a few thousand frontier
+    // nodes), so fewer workgroups mean fewer end-of-step tickets and a
+    // cheaper dispatch (1000x1000x12, 256 lanes: 128 groups 42 ms per query,
+    // 512 groups 55 ms, 1024 groups 76 ms).
+    const int grid = std::max(1, cus / 2);
     if (e == hipSuccess) e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         for (int i = 0; i < kGridSteps; ++i) {
