@@ -534,13 +534,21 @@ class DevicePathFinder:
             if graph and len(waves):
                 cs = C.c_void_p()
                 _lib.call("orx_graph_begin", C.byref(cs))
+                g = C.c_void_p()
                 try:
                     _lib.call("orx_route_waves", *route_args, cs)
-                finally:
-                    g = C.c_void_p()
-                    _lib.call("orx_graph_end", cs, C.byref(g))
+                except Exception:
+                    # end the capture (which frees the stream) and drop the graph
+                    if _lib.load().orx_graph_end(cs, C.byref(g)) == _lib.ORX_OK:
+                        _lib.load().orx_graph_destroy(g)
+                    raise
+                _lib.call("orx_graph_end", cs, C.byref(g))
                 exec_graph = g
-                _lib.call("orx_graph_launch", exec_graph, s)
+                try:
+                    _lib.call("orx_graph_launch", exec_graph, s)
+                except Exception:
+                    _lib.load().orx_graph_destroy(exec_graph)
+                    raise
             else:
                 _lib.call("orx_route_waves", *route_args, s)
             ev[2].record(stream)
