@@ -67,6 +67,10 @@ def load() -> C.CDLL:
     lib.orx_oracle_pathfinder.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, P, P, P, C.c_int32, P, P,
         C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int64]
+    lib.orx_oracle_pathfinder_waves.restype = C.c_int
+    lib.orx_oracle_pathfinder_waves.argtypes = [
+        C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, P, P, P, P, P, C.c_int32, C.c_float,
+        C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int32]
     lib.orx_oracle_sssp_box.restype = C.c_int
     lib.orx_oracle_sssp_box.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -177,6 +181,39 @@ class OraclePathFinder:
         d = {k: int(stats[i]) for i, k in enumerate(keys)}
         d["iteration"] = it
         d["pres_fac"] = float(pres[0])
+        self.history.append(d)
+        return d
+
+    def run_iteration_waves(self, threads: int = 0) -> dict:
+        """The same iteration as run_iteration, with the nets of each
+        disjoint-ROI wave routed on `threads` host threads (0 = all); the
+        result equals the sequential order's."""
+        from orthoroute_amd.waves import color_waves, group_waves
+        it = self.iteration + 1
+        order = self.order_for(it)
+        s = self.spec
+        waves = group_waves(color_waves(self.netlist.box, order, s.X, s.Y), order, self.netlist.roi_nodes())
+        sched = np.ascontiguousarray(np.concatenate(waves) if waves else np.zeros(0, np.int32), np.int32)
+        wave_ptr = np.zeros(len(waves) + 1, np.int32)
+        np.cumsum([len(w) for w in waves], out=wave_ptr[1:])
+        stats = np.zeros(16, np.int64)
+        P = params_from(self.cfg, 1.0)
+        P.node_pres = float(np.float32(self.cfg.node_pres_w))
+        P.astar_cp, P.astar_cv, P.astar_gmax = self.astar
+        rc = self.lib.orx_oracle_pathfinder_waves(
+            s.X, s.Y, s.L, s.hmask, _p(self.base), self.netlist.n, _p(self.src), _p(self.dst), _p(self.box),
+            _p(sched), _p(wave_ptr), len(waves), float(self.schedule[it - 1]), C.byref(P), _p(self.hist),
+            _p(self.occ), _p(self.node_hist), _p(self.node_use), _p(self.via_col), _p(self.arena),
+            _p(self.path_off), _p(self.path_cap), _p(self.path_len), _p(self.path_cost), _p(stats), int(threads))
+        if rc < 0:
+            raise RuntimeError(f"oracle wave pathfinder failed: {rc}")
+        self.iteration = it
+        keys = ["routed", "failed", "edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops",
+                "barrels", "settled", "relaxed", "path_hops"]
+        d = {k: int(stats[i]) for i, k in enumerate(keys)}
+        d["iteration"] = it
+        d["pres_fac"] = float(self.schedule[it - 1])
+        d["waves"] = len(waves)
         self.history.append(d)
         return d
 

@@ -39,6 +39,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 #include <math.h>
 
 #define KEY_INF 0xFFFFFFFFFFFFFFFFull
@@ -470,6 +471,33 @@ void orx_oracle_net_overuse(int32_t X, int32_t Y, int32_t L, uint32_t hmask, con
 /* dist_out[n] receives each net's final path cost (fp32) of the last        */
 /* iteration it was routed in.                                               */
 /* ------------------------------------------------------------------------ */
+/* The per-net body of _route_all (UP:7949-8430): rip up the old path,
+ * search, backtrace, commit.  Touches only arcs and nodes inside the net's
+ * box.  st[0] routed, st[1] failed, st[8..9] settled / relaxed, st[10] path
+ * hops.  Returns 0 or a negative status. */
+static int route_one(const lat_t* g, const lat_state* S, const orx_params* P, int32_t k,
+                     const int32_t* src, const int32_t* dst, const int32_t* box, int32_t* occ,
+                     int32_t* node_use, int32_t* via_col, int32_t* path_arena, const int64_t* path_off,
+                     const int64_t* path_cap, int32_t* path_len, float* dist_out, sssp_ws* W, int64_t* st) {
+    int32_t* pk = path_arena + path_off[k];
+    if (path_len[k] > 0) {
+        if (apply_path(g, pk, path_len[k], -1, occ, node_use, via_col)) return -7;
+        path_len[k] = 0;
+    }
+    const int32_t* b = box + 4 * (int64_t)k;
+    int r = box_route(g, S, P, b[0], b[1], b[2], b[3], src[k], dst[k], W, st + 8);
+    if (r < 0) return r;
+    if (r == 1) { st[1] += 1; return 0; }
+    int64_t len = box_backtrace(g, b[0], b[1], b[2], b[3], dst[k], W, pk, path_cap[k]);
+    if (len < 0) return (int)len;
+    path_len[k] = (int32_t)len;
+    if (dist_out) dist_out[k] = kdist(W->key[box_index(g, b[0], b[1], b[2], b[3], dst[k])]);
+    if (apply_path(g, pk, len, +1, occ, node_use, via_col)) return -7;
+    st[0] += 1;
+    st[10] += len - 1;
+    return 0;
+}
+
 int orx_oracle_pathfinder(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const float* base,
                           int32_t n_nets, const int32_t* src, const int32_t* dst, const int32_t* box,
                           int32_t iters, const int32_t* order, const float* pres,
@@ -494,22 +522,9 @@ int orx_oracle_pathfinder(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const
             if (budget == 0) goto done;
             if (budget > 0) --budget;
             int32_t k = order[(int64_t)it * n_nets + i];
-            int32_t* pk = path_arena + path_off[k];
-            if (path_len[k] > 0) {
-                if (apply_path(&g, pk, path_len[k], -1, occ, node_use, via_col)) { rc = -7; goto done; }
-                path_len[k] = 0;
-            }
-            const int32_t* b = box + 4 * (int64_t)k;
-            int r = box_route(&g, &S, &P, b[0], b[1], b[2], b[3], src[k], dst[k], &W, st + 8);
+            int r = route_one(&g, &S, &P, k, src, dst, box, occ, node_use, via_col, path_arena, path_off,
+                              path_cap, path_len, dist_out, &W, st);
             if (r < 0) { rc = r; goto done; }
-            if (r == 1) { st[1] += 1; continue; }
-            int64_t len = box_backtrace(&g, b[0], b[1], b[2], b[3], dst[k], &W, pk, path_cap[k]);
-            if (len < 0) { rc = (int)len; goto done; }
-            path_len[k] = (int32_t)len;
-            if (dist_out) dist_out[k] = kdist(W.key[box_index(&g, b[0], b[1], b[2], b[3], dst[k])]);
-            if (apply_path(&g, pk, len, +1, occ, node_use, via_col)) { rc = -7; goto done; }
-            st[0] += 1;
-            st[10] += len - 1;
         }
         {
             int64_t ov[6];
@@ -663,4 +678,58 @@ int orx_oracle_sssp_csr(int64_t N, const int32_t* indptr, const int32_t* indices
 out:
     free(key); free(done); free(is_tg); free(H.a);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* One PathFinder iteration with the nets of each disjoint-ROI wave routed   */
+/* on host threads (OpenMP): the all-core CPU baseline.  sched[n] lists the  */
+/* nets wave by wave, wave_ptr[n_waves+1] the wave boundaries.  Nets of one  */
+/* wave touch disjoint arcs and nodes, so the result equals the sequential   */
+/* order's.  stats[16] as one iteration of orx_oracle_pathfinder.            */
+/* ------------------------------------------------------------------------ */
+int orx_oracle_pathfinder_waves(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const float* base,
+                                int32_t n_nets, const int32_t* src, const int32_t* dst, const int32_t* box,
+                                const int32_t* sched, const int32_t* wave_ptr, int32_t n_waves, float pres_it,
+                                const orx_params* P0, float* hist, int32_t* occ, float* node_hist,
+                                int32_t* node_use, int32_t* via_col, int32_t* path_arena,
+                                const int64_t* path_off, const int64_t* path_cap, int32_t* path_len,
+                                float* dist_out, int64_t* stats, int32_t threads) {
+    lat_t g = { X, Y, L, hmask };
+    int64_t N = (int64_t)X * Y * L;
+    orx_params P = *P0;
+    P.pres_fac = pres_it;
+    P.node_pres = P0->node_pres * pres_it;
+    lat_state S = { base, hist, occ, node_hist, node_use, NULL };
+    int T = threads > 0 ? threads : omp_get_max_threads();
+    sssp_ws* W = (sssp_ws*)calloc((size_t)T, sizeof(sssp_ws));
+    int64_t* tst = (int64_t*)calloc((size_t)T * 16, sizeof(int64_t));
+    if (!W || !tst) { free(W); free(tst); return -8; }
+    int rc = 0;
+    (void)n_nets;
+    for (int32_t w = 0; w < n_waves && rc == 0; ++w) {
+        const int32_t a = wave_ptr[w], b = wave_ptr[w + 1];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(T)
+        for (int32_t i = a; i < b; ++i) {
+            const int t = omp_get_thread_num();
+            int r = route_one(&g, &S, &P, sched[i], src, dst, box, occ, node_use, via_col, path_arena,
+                              path_off, path_cap, path_len, dist_out, &W[t], tst + 16 * (int64_t)t);
+            if (r < 0) {
+#pragma omp atomic write
+                rc = r;
+            }
+        }
+    }
+    memset(stats, 0, 16 * sizeof(int64_t));
+    for (int t = 0; t < T; ++t) {
+        for (int j = 0; j < 16; ++j) stats[j] += tst[16 * (int64_t)t + j];
+        ws_free(&W[t]);
+    }
+    free(W);
+    free(tst);
+    if (rc < 0) return rc;
+    int64_t ov[6];
+    orx_oracle_overuse(X, Y, L, occ, node_use, via_col, P.edge_cap, P.node_cap, ov);
+    for (int j = 0; j < 6; ++j) stats[2 + j] = ov[j];
+    orx_oracle_update_history(4 * N, base, hist, occ, N, node_hist, node_use, &P);
+    return 0;
 }

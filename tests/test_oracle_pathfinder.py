@@ -81,3 +81,23 @@ def test_overuse_falls_on_plain_64():
     o = OraclePathFinder(plain_64(), RouterConfig(iterations=10))
     over = [(d := o.run_iteration())["edge_over"] + d["node_over"] for _ in range(10)]
     assert over[-1] < over[0] / 50
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_wave_parallel_oracle_equals_sequential(threads):
+    """The all-core CPU baseline (nets of a disjoint-ROI wave on host
+    threads) gives the sequential order's result, bit for bit."""
+    from orthoroute_amd.netgen import uneven
+    nl = uneven(90, 70, 6, 260, seed=17)
+    cfg = RouterConfig(iterations=3)
+    a, b = OraclePathFinder(nl, cfg), OraclePathFinder(nl, cfg)
+    for _ in range(3):
+        da = a.run_iteration()
+        db = b.run_iteration_waves(threads)
+        for k in ("routed", "failed", "edge_over", "node_over", "via_hops", "barrels", "settled", "relaxed",
+                  "path_hops"):
+            assert da[k] == db[k], k
+        for x, y in zip(a.paths(), b.paths()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(a.occ, b.occ)
+        assert np.array_equal(a.hist.view(np.uint32), b.hist.view(np.uint32))
