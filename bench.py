@@ -97,6 +97,29 @@ def cpu_baseline_all_cores(nl, cfg) -> dict:
                       f"threads, {dt:.1f} s on {_cpu_model()}", "seconds": dt}
 
 
+def verify_lockstep(nl, cfg, iterations: int) -> dict:
+    """Batched waves against the sequential per-net order (plain one net per
+    launch) for `iterations` iterations: per-iteration counts and the digest
+    of paths, fp32 costs, occupancy and node use must agree."""
+    import dataclasses
+
+    from orthoroute_amd.router import DevicePathFinder
+    c = dataclasses.replace(cfg, iterations=iterations, collect_stats=False, stop_at_zero_overuse=False)
+    seq, bat = DevicePathFinder(nl, c), DevicePathFinder(nl, c)
+    rows = []
+    for it in range(iterations):
+        ss, sb = seq.run_iteration("plain"), bat.run_iteration("batched")
+        same = (ss.overuse == sb.overuse and ss.routed == sb.routed and ss.via_hops == sb.via_hops
+                and ss.path_hops == sb.path_hops and seq.digest() == bat.digest())
+        rows.append({"iteration": it + 1, "overuse": sb.overuse, "plain_ms": round(ss.seconds * 1e3, 1),
+                     "batched_ms": round(sb.seconds * 1e3, 1), "bit_exact": same})
+        _log(f"lockstep {nl.name} iteration {it + 1}: overuse {sb.overuse}, bit-exact {same}")
+        if not same:
+            break
+    return {"workload": nl.name, "iterations": len(rows), "all_bit_exact": all(r["bit_exact"] for r in rows),
+            "rows": rows}
+
+
 def copy_probe(mib: int = 256, reps: int = 5) -> None:
     import torch
     n = mib * (1 << 20) // 4
@@ -232,6 +255,8 @@ def main() -> int:
     ap.add_argument("--serve-seed", type=int, default=4)
     ap.add_argument("--graph", action="store_true", help="capture each iteration's wave launches in a hipGraph")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--verify-lockstep", type=int, default=0, metavar="N",
+                    help="only run N iterations of batched vs sequential order on --workload and print the result")
     ap.add_argument("--delta", type=float, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--lds-nodes", type=int, default=None)
@@ -252,6 +277,11 @@ def main() -> int:
     _lib.init_device()
     nl = getattr(netgen, args.workload)()
     tuned = load_tuned(nl)
+    if args.verify_lockstep:
+        res = verify_lockstep(nl, RouterConfig(delta=tuned["delta"], block=tuned["block"],
+                                               lds_nodes=tuned["lds_nodes"]), args.verify_lockstep)
+        print(json.dumps(res))
+        return 0 if res["all_bit_exact"] else 1
     delta = args.delta if args.delta is not None else tuned["delta"]
     block = args.block if args.block is not None else tuned["block"]
     lds_nodes = args.lds_nodes if args.lds_nodes is not None else tuned["lds_nodes"]
