@@ -226,6 +226,44 @@ def serving_bench(n_jobs: int, rate: float, seed: int, max_iterations: int, tune
     return summ
 
 
+def portal_bench(nl, cfg, candidates: int, steps: int, cpu_stride: int, with_cpu: bool) -> dict:
+    """The workload in portal form (every pad a seed or target on each inner
+    layer of ``candidates`` escape cells): one warm-up and ``steps`` timed
+    batched iterations, and the oracle on one core on every
+    ``cpu_stride``-th net of iteration 1."""
+    import dataclasses
+
+    import torch
+    from orthoroute_amd.netgen import with_portals
+    from orthoroute_amd.router import DevicePathFinder
+    pnl = with_portals(nl, candidates=candidates, seed=0)
+    ns, nt = pnl.terms.counts()
+    c = dataclasses.replace(cfg, iterations=1 + steps, collect_stats=False)
+    dev = DevicePathFinder(pnl, c)
+    warm = dev.run_iteration("batched")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    timed = [dev.run_iteration("batched") for _ in range(steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": pnl.name, "seeds_per_net": float(ns.mean()), "targets_per_net": float(nt.mean()),
+           "value": sum(s.routed for s in timed) / wall, "unit": "nets/s", "ms_per_iteration": wall / steps * 1e3,
+           "iteration1_ms": warm.seconds * 1e3, "overuse_last": timed[-1].overuse,
+           "waves_per_iteration": sum(s.waves for s in timed) / steps}
+    del dev
+    torch.cuda.empty_cache()
+    if with_cpu:
+        from oracle.oracle import OraclePathFinder
+        sub = pnl.select(list(range(0, pnl.n, cpu_stride)), f"::{cpu_stride}")
+        o = OraclePathFinder(sub, dataclasses.replace(cfg, iterations=1))
+        t0 = time.perf_counter()
+        d = o.run_iteration()
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": d["routed"] / dt, "unit": "nets/s", "cores": 1, "kind": "port",
+                               "sample": f"iteration 1 of every {cpu_stride}th net ({sub.n} nets), {dt:.1f} s"}
+    return out
+
+
 def _traffic():
     """HBM bytes per route_wave_kernel dispatch from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
@@ -253,6 +291,10 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-iterations", type=int, default=50)
     ap.add_argument("--serve-seed", type=int, default=4)
+    ap.add_argument("--no-portal", action="store_true")
+    ap.add_argument("--portal-candidates", type=int, default=1)
+    ap.add_argument("--portal-steps", type=int, default=2)
+    ap.add_argument("--portal-cpu-stride", type=int, default=4)
     ap.add_argument("--graph", action="store_true", help="capture each iteration's wave launches in a hipGraph")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--verify-lockstep", type=int, default=0, metavar="N",
@@ -357,6 +399,13 @@ def main() -> int:
                                 time_limit=300.0)
         torch.cuda.empty_cache()
 
+    portal = None
+    if not args.no_portal and nl.spec.L >= 3:
+        _log(f"portal form: {args.portal_candidates} escape cell(s) per pad, {args.portal_steps} timed iterations")
+        portal = portal_bench(nl, cfg, args.portal_candidates, args.portal_steps, args.portal_cpu_stride,
+                              not args.no_cpu)
+        _log(f"portal form: {portal['ms_per_iteration']:.1f} ms/iteration")
+
     cpu = cpu_all = None
     if not args.no_cpu:
         _log("CPU baseline (oracle, 1 core)")
@@ -400,6 +449,7 @@ def main() -> int:
         "cpu_baseline_all_cores": cpu_all,
         "sssp_microbench": sssp,
         "serving": serving,
+        "portal_nets": portal,
     }
     print(json.dumps(out))
     return 0

@@ -76,7 +76,8 @@ typedef struct orx_params {
     /* A* toward the sink in orx_route_wave (0 = plain Dijkstra order):
      * per-move lower bounds of the entering weight, planar / via, each at
      * least half an fp32 ulp of astar_gmax below the true minimum; results
-     * with a sink distance >= astar_gmax are recomputed without it. */
+     * with a sink distance (multi-terminal nets: a chosen total) >=
+     * astar_gmax are recomputed without it. */
     float astar_cp;
     float astar_cv;
     float astar_gmax;
@@ -94,7 +95,28 @@ typedef struct orx_nets {
     int32_t* path_len;        /* [n] 0 = unrouted                           */
     float* path_cost;         /* [n] fp32 distance of the sink              */
     int32_t* status;          /* [n] last status (ORX_OK / ORX_NO_PATH / <0) */
+    /* Multi-terminal nets (portal seeds and targets,
+     * find_path_multisource_multisink UP:2550-2659 with the seeds of
+     * _get_pad_portal_seeds UP:4054-4180), or NULL seed_ptr for none.  Net k
+     * with seed_ptr[k+1] > seed_ptr[k] ignores src/dst and routes from the
+     * seeds seed_node[seed_ptr[k] .. seed_ptr[k+1]), each starting at
+     * fl(seed_cost + the seed's node term) (CD:5589-5606; duplicates keep the
+     * minimum), to the reached target of tgt_node[tgt_ptr[k] ..
+     * tgt_ptr[k+1]) with the smallest (fl(dist + tgt_cost), node id).
+     * Targets are not expanded (UP:2611-2618); 1..ORX_MAX_TARGETS per net;
+     * terminals outside the net's box are ignored.  path_cost is then that
+     * total.  A best target that is itself a seed gives ORX_NO_PATH
+     * (a one-node path, UP:2647-2648).  Costs are >= 0 (seed costs
+     * finite); tgt_cost may be NULL (all 0). */
+    const int32_t* seed_ptr;  /* [n+1] */
+    const int32_t* seed_node; /* global node ids                             */
+    const float* seed_cost;
+    const int32_t* tgt_ptr;   /* [n+1] */
+    const int32_t* tgt_node;
+    const float* tgt_cost;
 } orx_nets;
+
+#define ORX_MAX_TARGETS 128
 
 /* Scratch for one wave: every slot k of the wave owns nodes
  * [ws_off[k], ws_off[k] + R_k) of keys (u64), 4 queues (u32, stride
@@ -124,8 +146,9 @@ typedef struct orx_route_cfg {
                             /* seed forms UP:2582-2583, CD:5589-5606)        */
                             /* (bits 16 and up are reserved)                 */
 
-/* stats[]: 0 settled nodes (label-setting count up to the sink), 1 relaxed
- * arcs (their in-ROI out-degree), 2 queue items processed, 3 near rounds,
+/* stats[]: 0 settled nodes (label-setting count up to the sink; for a
+ * multi-terminal net up to the chosen target), 1 relaxed arcs (their in-ROI
+ * out-degree, targets excluded), 2 queue items processed, 3 near rounds,
  * 4 threshold advances, 5 nets routed, 6 nets failed, 7 path hops. */
 
 /* Library init: checks for a gfx950 device.  Replaces the CuPy

@@ -47,6 +47,22 @@ class OracleParams(C.Structure):
     ]
 
 
+class OracleTerms(C.Structure):
+    """Seeds and targets of multi-terminal nets (orx_terms in orx_oracle.c)."""
+    _fields_ = [("seed_ptr", C.c_void_p), ("seed_node", C.c_void_p), ("seed_cost", C.c_void_p),
+                ("tgt_ptr", C.c_void_p), ("tgt_node", C.c_void_p), ("tgt_cost", C.c_void_p)]
+
+
+def terms_struct(terms):
+    """(ctypes struct or None, arrays kept alive) for a netlist's Terminals."""
+    if terms is None:
+        return None, ()
+    keep = (np.ascontiguousarray(terms.seed_ptr, np.int32), np.ascontiguousarray(terms.seed_node, np.int32),
+            np.ascontiguousarray(terms.seed_cost, np.float32), np.ascontiguousarray(terms.tgt_ptr, np.int32),
+            np.ascontiguousarray(terms.tgt_node, np.int32), np.ascontiguousarray(terms.tgt_cost, np.float32))
+    return OracleTerms(*(a.ctypes.data for a in keep)), keep
+
+
 _lib = None
 
 
@@ -66,11 +82,11 @@ def load() -> C.CDLL:
     lib.orx_oracle_pathfinder.restype = C.c_int
     lib.orx_oracle_pathfinder.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, P, P, P, C.c_int32, P, P,
-        C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int64]
+        C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int64, C.POINTER(OracleTerms)]
     lib.orx_oracle_pathfinder_waves.restype = C.c_int
     lib.orx_oracle_pathfinder_waves.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, P, P, P, P, P, C.c_int32, C.c_float,
-        C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int32]
+        C.POINTER(OracleParams), P, P, P, P, P, P, P, P, P, P, P, C.c_int32, C.POINTER(OracleTerms)]
     lib.orx_oracle_sssp_box.restype = C.c_int
     lib.orx_oracle_sssp_box.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -138,6 +154,7 @@ class OraclePathFinder:
         self.astar = cfg.astar_bounds(self.base)
         self.iteration = 0
         self.history = []
+        self.terms, self._terms_keep = terms_struct(getattr(netlist, "terms", None))
 
     def net_overuse(self):
         n = self.netlist.n
@@ -172,7 +189,8 @@ class OraclePathFinder:
             s.X, s.Y, s.L, s.hmask, _p(self.base), self.netlist.n, _p(self.src), _p(self.dst), _p(self.box),
             1, _p(order), _p(pres), C.byref(P), _p(self.hist), _p(self.occ), _p(self.node_hist),
             _p(self.node_use), _p(self.via_col), _p(self.arena), _p(self.path_off), _p(self.path_cap),
-            _p(self.path_len), _p(self.path_cost), _p(stats), int(nets_limit))
+            _p(self.path_len), _p(self.path_cost), _p(stats), int(nets_limit),
+            None if self.terms is None else C.byref(self.terms))
         if rc < 0:
             raise RuntimeError(f"oracle pathfinder failed: {rc}")
         self.iteration = it
@@ -204,7 +222,8 @@ class OraclePathFinder:
             s.X, s.Y, s.L, s.hmask, _p(self.base), self.netlist.n, _p(self.src), _p(self.dst), _p(self.box),
             _p(sched), _p(wave_ptr), len(waves), float(self.schedule[it - 1]), C.byref(P), _p(self.hist),
             _p(self.occ), _p(self.node_hist), _p(self.node_use), _p(self.via_col), _p(self.arena),
-            _p(self.path_off), _p(self.path_cap), _p(self.path_len), _p(self.path_cost), _p(stats), int(threads))
+            _p(self.path_off), _p(self.path_cap), _p(self.path_len), _p(self.path_cost), _p(stats), int(threads),
+            None if self.terms is None else C.byref(self.terms))
         if rc < 0:
             raise RuntimeError(f"oracle wave pathfinder failed: {rc}")
         self.iteration = it

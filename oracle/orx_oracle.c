@@ -199,7 +199,7 @@ static inline float arc_weight(const lat_t* g, const lat_state* S, const orx_par
 }
 
 typedef struct {
-    uint64_t* key; uint8_t* done; heap_t heap; int64_t cap;
+    uint64_t* key; uint8_t* done; uint8_t* tmark; heap_t heap; int64_t cap;
 } sssp_ws;
 
 /* ROI-local index of global node gid in box (x0,y0,w,h), -1 when outside */
@@ -214,33 +214,41 @@ static inline int64_t box_index(const lat_t* g, int32_t x0, int32_t y0, int32_t 
 
 static int ws_reserve(sssp_ws* W, int64_t R) {
     if (R <= W->cap) return 0;
-    free(W->key); free(W->done);
+    free(W->key); free(W->done); free(W->tmark);
     W->key = (uint64_t*)malloc((size_t)R * 8);
     W->done = (uint8_t*)malloc((size_t)R);
-    if (!W->key || !W->done) return -1;
+    W->tmark = (uint8_t*)calloc((size_t)R, 1);
+    if (!W->key || !W->done || !W->tmark) return -1;
     W->cap = R;
     return 0;
 }
-static void ws_free(sssp_ws* W) { free(W->key); free(W->done); free(W->heap.a); memset(W, 0, sizeof(*W)); }
+static void ws_free(sssp_ws* W) {
+    free(W->key); free(W->done); free(W->tmark); free(W->heap.a);
+    memset(W, 0, sizeof(*W));
+}
 
-/* A* heuristic toward the sink (tx, ty, tz):
- *   h(v) = cp*(|x-tx| + |y-ty|) + cv*|z-tz|, evaluated exactly in double.
+/* A* heuristic toward the sink box [tx0,tx1] x [ty0,ty1] x [tz0,tz1] (the
+ * sink itself for a two-pin net, the targets' bounding box for a
+ * multi-terminal net):
+ *   h(v) = cp*(dx + dy) + cv*dz, d* = distance to the box along that axis,
+ * evaluated exactly in double; h = 0 on every sink.
  * cp / cv are per-move lower bounds of the entering weight minus a margin
  * m >= half an fp32 ulp below gmax, so for every tight arc u->v with
  * g(v) < gmax, f(u) = g(u)+h(u) <= g(v)+h(v) = f(v): popping by f settles
- * exactly the canonical keys.  A result with g(dst) >= gmax is recomputed
- * without the heuristic (box_route). */
-typedef struct { int on; double cp, cv; int32_t tx, ty, tz; } astar_t;
+ * exactly the canonical keys.  A result with a sink at g >= gmax is
+ * recomputed without the heuristic (box_route, box_route_multi). */
+typedef struct { int on; double cp, cv; int32_t tx0, tx1, ty0, ty1, tz0, tz1; } astar_t;
 
+static inline int32_t axis_gap(int32_t v, int32_t lo, int32_t hi) {
+    return v < lo ? lo - v : (v > hi ? v - hi : 0);
+}
 static inline double heur(const astar_t* A, const lat_t* g, int64_t gid) {
     if (!A || !A->on) return 0.0;
     int64_t XY = (int64_t)g->X * g->Y;
     int32_t z = (int32_t)(gid / XY);
     int64_t rem = gid - (int64_t)z * XY;
     int32_t y = (int32_t)(rem / g->X), x = (int32_t)(rem - (int64_t)y * g->X);
-    int32_t dx = x > A->tx ? x - A->tx : A->tx - x;
-    int32_t dy = y > A->ty ? y - A->ty : A->ty - y;
-    int32_t dz = z > A->tz ? z - A->tz : A->tz - z;
+    int32_t dx = axis_gap(x, A->tx0, A->tx1), dy = axis_gap(y, A->ty0, A->ty1), dz = axis_gap(z, A->tz0, A->tz1);
     return A->cp * (double)(dx + dy) + A->cv * (double)dz;
 }
 
@@ -312,9 +320,9 @@ static int box_route(const lat_t* g, const lat_state* S, const orx_params* P,
         const int64_t XY = (int64_t)g->X * g->Y;
         astar_t A;
         A.on = 1; A.cp = P->astar_cp; A.cv = P->astar_cv;
-        A.tz = (int32_t)(dst / XY);
-        int64_t rem = dst - (int64_t)A.tz * XY;
-        A.ty = (int32_t)(rem / g->X); A.tx = (int32_t)(rem - (int64_t)A.ty * g->X);
+        A.tz0 = A.tz1 = (int32_t)(dst / XY);
+        int64_t rem = dst - (int64_t)A.tz0 * XY;
+        A.ty0 = A.ty1 = (int32_t)(rem / g->X); A.tx0 = A.tx1 = (int32_t)(rem - (int64_t)A.ty0 * g->X);
         int r = box_sssp(g, S, P, x0, y0, w, h, src, dst, 0, W, st, &A);
         if (r < 0 || r == 1) { stats[0] += st[0]; stats[1] += st[1]; return r; }
         uint64_t kd = W->key[box_index(g, x0, y0, w, h, dst)];
@@ -353,6 +361,222 @@ static int64_t box_backtrace(const lat_t* g, int32_t x0, int32_t y0, int32_t w, 
         if (v < 0) return -6;
     }
     return len;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-terminal nets: find_path_multisource_multisink                      */
+/* (unified_pathfinder.py:2550-2659) on the net's box.  Every seed inside    */
+/* the box starts at fl(seed_cost + q_seed), the seed paying its own node    */
+/* term (UP:2582-2583, cuda_dijkstra.py:5589-5606; a seed listed twice keeps */
+/* the smaller key); targets are popped but never expanded (UP:2611-2618);   */
+/* the search stops once every target in the box is popped.  The chosen     */
+/* target minimises (fl(dist + target_cost), node id) over reached targets   */
+/* with a finite total (UP:2612-2617 keeps the first strict improvement).    */
+/* A one-node result (the best target is a seed) is no path (UP:2647-2648). */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const int32_t* seed_ptr;   /* [n+1]; NULL = every net is two-pin        */
+    const int32_t* seed_node;
+    const float* seed_cost;
+    const int32_t* tgt_ptr;    /* [n+1] */
+    const int32_t* tgt_node;
+    const float* tgt_cost;     /* NULL = all 0 */
+} orx_terms;
+
+#define MAX_TARGETS 128
+
+/* Smallest fp32 value >= x. */
+static inline float f32_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+/* The search pops in (f, key, node) order and stops before popping a node of
+ * f = F once the best certified target cannot be beaten: a target popped
+ * with dist < F' = min(F, gmax) is certified (exact), and every other target
+ * has a true dist >= F', so its total is at least fl(F'^ + tc) with F'^ the
+ * smallest fp32 >= F'; the search stops when that bound exceeds the best
+ * certified total for all of them (gmax = +inf without the heuristic).
+ * Returns 0 when the search ran (targets reached or not), 1 when no seed or
+ * no target lies in the box, <0 on error. */
+static int box_sssp_multi(const lat_t* g, const lat_state* S, const orx_params* P,
+                          int32_t x0, int32_t y0, int32_t w, int32_t h,
+                          int32_t ns, const int32_t* sn, const float* sc, int32_t nt, const int32_t* tn,
+                          const float* tc, sssp_ws* W, const astar_t* A, double gmax) {
+    const int64_t XY = (int64_t)g->X * g->Y;
+    const int64_t R = (int64_t)w * h * g->L;
+    if (ws_reserve(W, R)) return -2;
+    for (int64_t r = 0; r < R; ++r) { W->key[r] = KEY_INF; W->done[r] = 0; W->tmark[r] = 0; }
+    W->heap.n = 0;
+    int64_t tr[MAX_TARGETS];
+    uint8_t cert[MAX_TARGETS];
+    int in_box = 0;
+    for (int32_t i = 0; i < nt; ++i) {
+        tr[i] = box_index(g, x0, y0, w, h, tn[i]);
+        cert[i] = 0;
+        if (tr[i] >= 0) { W->tmark[tr[i]] = 1; in_box = 1; }
+    }
+    int seeded = 0;
+    for (int32_t i = 0; i < ns; ++i) {
+        int64_t r = box_index(g, x0, y0, w, h, sn[i]);
+        if (r < 0) continue;
+        float c = sc[i] + node_cost(S->node_hist[sn[i]], S->node_use[sn[i]], P);
+        uint64_t k = kpack(c, 0, ROOT_SLOT);
+        seeded = 1;
+        if (k < W->key[r]) {
+            W->key[r] = k;
+            if (heap_push(&W->heap, (double)c + heur(A, g, sn[i]), k, (int32_t)r)) return -2;
+        }
+    }
+    if (!seeded || !in_box) return 1;
+    float best = INFINITY;     /* best certified total */
+    int have = 0;
+    float tcmin = INFINITY;    /* smallest cost among uncertified targets */
+    for (int32_t i = 0; i < nt; ++i)
+        if (tr[i] >= 0) tcmin = fminf(tcmin, tc ? tc[i] : 0.0f);
+    const int64_t wh = (int64_t)w * h;
+    while (W->heap.n) {
+        hent e = heap_pop(&W->heap);
+        if (W->done[e.v] || e.k != W->key[e.v]) continue;
+        if (have) {
+            const float lb = f32_up(e.f < gmax ? e.f : gmax) + tcmin;
+            if (lb > best) break;
+        }
+        W->done[e.v] = 1;
+        if (W->tmark[e.v]) {
+            const float d = kdist(e.k);
+            if ((double)d < gmax) {
+                tcmin = INFINITY;
+                for (int32_t i = 0; i < nt; ++i) {
+                    if (tr[i] < 0) continue;
+                    const float c = tc ? tc[i] : 0.0f;
+                    if (tr[i] == e.v) {
+                        cert[i] = 1;
+                        const float t = d + c;
+                        if (t < INFINITY && t < best) { best = t; have = 1; }
+                    }
+                    if (!cert[i]) tcmin = fminf(tcmin, c);
+                }
+            }
+            continue;   /* targets are sinks */
+        }
+        int32_t z = (int32_t)(e.v / wh);
+        int64_t rm = e.v - z * wh;
+        int32_t ly = (int32_t)(rm / w), lx = (int32_t)(rm - (int64_t)ly * w);
+        int64_t gu = (int64_t)z * XY + (int64_t)(y0 + ly) * g->X + (x0 + lx);
+        float du = kdist(e.k);
+        uint32_t hu = khops(e.k);
+        for (int s = 0; s < 4; ++s) {
+            int64_t gv = lat_nbr(g, gu, s);
+            if (gv < 0) continue;
+            int64_t rv = box_index(g, x0, y0, w, h, gv);
+            if (rv < 0 || W->done[rv]) continue;
+            float wt = arc_weight(g, S, P, gu, s, gv);
+            if (isinf(wt)) continue;
+            float nd = du + wt;
+            if (hu + 1 > HOPS_MAX) return -4;
+            uint64_t nk = kpack(nd, hu + 1, (uint32_t)(3 - s));
+            if (nk < W->key[rv]) {
+                W->key[rv] = nk;
+                if (heap_push(&W->heap, (double)nd + heur(A, g, gv), nk, (int32_t)rv)) return -2;
+            }
+        }
+    }
+    return 0;
+}
+
+/* Best popped target, (fl(dist + tc), node id) over those with a finite
+ * total: its global id (-1 when none) and total. */
+static void pick_target(const lat_t* g, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t nt,
+                        const int32_t* tn, const float* tc, const sssp_ws* W, int64_t* best, float* total) {
+    *best = -1;
+    *total = INFINITY;
+    for (int32_t i = 0; i < nt; ++i) {
+        int64_t r = box_index(g, x0, y0, w, h, tn[i]);
+        if (r < 0 || !W->done[r]) continue;
+        float t = kdist(W->key[r]) + (tc ? tc[i] : 0.0f);
+        if (!(t < INFINITY)) continue;
+        if (t < *total || (t == *total && tn[i] < *best)) { *total = t; *best = tn[i]; }
+    }
+}
+
+/* Search counts of a finished multi-terminal search, defined by the keys
+ * alone: the nodes a label-setting search popping in (f, key, node) order
+ * settles up to and including the chosen target (every reached node when
+ * there is none), and the in-box out-degree of those that are not targets. */
+static void count_upto(const lat_t* g, int32_t x0, int32_t y0, int32_t w, int32_t h, const sssp_ws* W,
+                       const astar_t* A, int64_t best, int64_t* stats) {
+    const int64_t XY = (int64_t)g->X * g->Y, wh = (int64_t)w * h, R = wh * g->L;
+    double fb = INFINITY;
+    uint64_t kb = KEY_INF;
+    int64_t rb = -1;
+    if (best >= 0) {
+        rb = box_index(g, x0, y0, w, h, best);
+        kb = W->key[rb];
+        fb = (double)kdist(kb) + heur(A, g, best);
+    }
+    for (int64_t r = 0; r < R; ++r) {
+        uint64_t k = W->key[r];
+        if (k == KEY_INF) continue;
+        int32_t z = (int32_t)(r / wh);
+        int64_t rm = r - z * wh;
+        int32_t ly = (int32_t)(rm / w), lx = (int32_t)(rm - (int64_t)ly * w);
+        int64_t gid = (int64_t)z * XY + (int64_t)(y0 + ly) * g->X + (x0 + lx);
+        if (best >= 0) {
+            double f = (double)kdist(k) + heur(A, g, gid);
+            if (f > fb || (f == fb && (k > kb || (k == kb && r > rb)))) continue;
+        }
+        stats[0] += 1;
+        if (W->tmark[r]) continue;
+        for (int s = 0; s < 4; ++s) {
+            int64_t gv = lat_nbr(g, gid, s);
+            if (gv >= 0 && box_index(g, x0, y0, w, h, gv) >= 0) stats[1] += 1;
+        }
+    }
+}
+
+/* Multi-terminal route: A* toward the targets' bounding box when P enables
+ * it, redone without the heuristic when the chosen total is >= astar_gmax
+ * (below it the chosen target, its total and its parent chain are exact:
+ * the true best total is no larger, so the true best target lies below
+ * gmax and is certified).  Returns 0 routed (best, total set), 1 no path,
+ * <0 error. */
+static int box_route_multi(const lat_t* g, const lat_state* S, const orx_params* P,
+                           int32_t x0, int32_t y0, int32_t w, int32_t h,
+                           int32_t ns, const int32_t* sn, const float* sc, int32_t nt, const int32_t* tn,
+                           const float* tc, sssp_ws* W, int64_t* stats, int64_t* best, float* total) {
+    const int64_t XY = (int64_t)g->X * g->Y;
+    if (P->astar_cp > 0.0f && P->astar_cv > 0.0f && P->astar_gmax > 0.0f) {
+        astar_t A;
+        A.on = 1; A.cp = P->astar_cp; A.cv = P->astar_cv;
+        A.tx0 = A.ty0 = A.tz0 = INT32_MAX;
+        A.tx1 = A.ty1 = A.tz1 = INT32_MIN;
+        for (int32_t i = 0; i < nt; ++i) {
+            if (box_index(g, x0, y0, w, h, tn[i]) < 0) continue;
+            int32_t z = (int32_t)(tn[i] / XY);
+            int64_t rem = tn[i] - (int64_t)z * XY;
+            int32_t y = (int32_t)(rem / g->X), x = (int32_t)(rem - (int64_t)y * g->X);
+            if (x < A.tx0) A.tx0 = x;
+            if (x > A.tx1) A.tx1 = x;
+            if (y < A.ty0) A.ty0 = y;
+            if (y > A.ty1) A.ty1 = y;
+            if (z < A.tz0) A.tz0 = z;
+            if (z > A.tz1) A.tz1 = z;
+        }
+        int r = box_sssp_multi(g, S, P, x0, y0, w, h, ns, sn, sc, nt, tn, tc, W, &A, (double)P->astar_gmax);
+        if (r != 0) return r;
+        pick_target(g, x0, y0, w, h, nt, tn, tc, W, best, total);
+        if (*best < 0 || *total < P->astar_gmax) {
+            count_upto(g, x0, y0, w, h, W, &A, *best, stats);
+            return *best < 0 ? 1 : 0;
+        }
+    }
+    int r = box_sssp_multi(g, S, P, x0, y0, w, h, ns, sn, sc, nt, tn, tc, W, NULL, INFINITY);
+    if (r != 0) return r;
+    pick_target(g, x0, y0, w, h, nt, tn, tc, W, best, total);
+    count_upto(g, x0, y0, w, h, W, NULL, *best, stats);
+    return *best < 0 ? 1 : 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -476,15 +700,37 @@ void orx_oracle_net_overuse(int32_t X, int32_t Y, int32_t L, uint32_t hmask, con
  * box.  st[0] routed, st[1] failed, st[8..9] settled / relaxed, st[10] path
  * hops.  Returns 0 or a negative status. */
 static int route_one(const lat_t* g, const lat_state* S, const orx_params* P, int32_t k,
-                     const int32_t* src, const int32_t* dst, const int32_t* box, int32_t* occ,
-                     int32_t* node_use, int32_t* via_col, int32_t* path_arena, const int64_t* path_off,
-                     const int64_t* path_cap, int32_t* path_len, float* dist_out, sssp_ws* W, int64_t* st) {
+                     const int32_t* src, const int32_t* dst, const int32_t* box, const orx_terms* T,
+                     int32_t* occ, int32_t* node_use, int32_t* via_col, int32_t* path_arena,
+                     const int64_t* path_off, const int64_t* path_cap, int32_t* path_len, float* dist_out,
+                     sssp_ws* W, int64_t* st) {
     int32_t* pk = path_arena + path_off[k];
     if (path_len[k] > 0) {
         if (apply_path(g, pk, path_len[k], -1, occ, node_use, via_col)) return -7;
         path_len[k] = 0;
     }
     const int32_t* b = box + 4 * (int64_t)k;
+    if (T && T->seed_ptr && T->seed_ptr[k + 1] > T->seed_ptr[k]) {
+        const int32_t s0 = T->seed_ptr[k], ns = T->seed_ptr[k + 1] - s0;
+        const int32_t t0 = T->tgt_ptr ? T->tgt_ptr[k] : 0;
+        const int32_t nt = T->tgt_ptr ? T->tgt_ptr[k + 1] - t0 : 0;
+        if (nt < 1 || nt > MAX_TARGETS) return -3;
+        int64_t best;
+        float total;
+        int r = box_route_multi(g, S, P, b[0], b[1], b[2], b[3], ns, T->seed_node + s0, T->seed_cost + s0, nt,
+                                T->tgt_node + t0, T->tgt_cost ? T->tgt_cost + t0 : NULL, W, st + 8, &best,
+                                &total);
+        if (r < 0) return r;
+        int64_t len = r == 1 ? 0 : box_backtrace(g, b[0], b[1], b[2], b[3], best, W, pk, path_cap[k]);
+        if (len < 0) return (int)len;
+        if (len < 2) { st[1] += 1; return 0; }
+        path_len[k] = (int32_t)len;
+        if (dist_out) dist_out[k] = total;
+        if (apply_path(g, pk, len, +1, occ, node_use, via_col)) return -7;
+        st[0] += 1;
+        st[10] += len - 1;
+        return 0;
+    }
     int r = box_route(g, S, P, b[0], b[1], b[2], b[3], src[k], dst[k], W, st + 8);
     if (r < 0) return r;
     if (r == 1) { st[1] += 1; return 0; }
@@ -505,7 +751,7 @@ int orx_oracle_pathfinder(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const
                           float* hist, int32_t* occ, float* node_hist, int32_t* node_use,
                           int32_t* via_col, int32_t* path_arena, const int64_t* path_off,
                           const int64_t* path_cap, int32_t* path_len, float* dist_out,
-                          int64_t* stats, int64_t nets_limit) {
+                          int64_t* stats, int64_t nets_limit, const orx_terms* terms) {
     lat_t g = { X, Y, L, hmask };
     int64_t N = (int64_t)X * Y * L;
     sssp_ws W; memset(&W, 0, sizeof(W));
@@ -522,7 +768,7 @@ int orx_oracle_pathfinder(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const
             if (budget == 0) goto done;
             if (budget > 0) --budget;
             int32_t k = order[(int64_t)it * n_nets + i];
-            int r = route_one(&g, &S, &P, k, src, dst, box, occ, node_use, via_col, path_arena, path_off,
+            int r = route_one(&g, &S, &P, k, src, dst, box, terms, occ, node_use, via_col, path_arena, path_off,
                               path_cap, path_len, dist_out, &W, st);
             if (r < 0) { rc = r; goto done; }
         }
@@ -693,7 +939,7 @@ int orx_oracle_pathfinder_waves(int32_t X, int32_t Y, int32_t L, uint32_t hmask,
                                 const orx_params* P0, float* hist, int32_t* occ, float* node_hist,
                                 int32_t* node_use, int32_t* via_col, int32_t* path_arena,
                                 const int64_t* path_off, const int64_t* path_cap, int32_t* path_len,
-                                float* dist_out, int64_t* stats, int32_t threads) {
+                                float* dist_out, int64_t* stats, int32_t threads, const orx_terms* terms) {
     lat_t g = { X, Y, L, hmask };
     int64_t N = (int64_t)X * Y * L;
     orx_params P = *P0;
@@ -711,7 +957,7 @@ int orx_oracle_pathfinder_waves(int32_t X, int32_t Y, int32_t L, uint32_t hmask,
 #pragma omp parallel for schedule(dynamic, 1) num_threads(T)
         for (int32_t i = a; i < b; ++i) {
             const int t = omp_get_thread_num();
-            int r = route_one(&g, &S, &P, sched[i], src, dst, box, occ, node_use, via_col, path_arena,
+            int r = route_one(&g, &S, &P, sched[i], src, dst, box, terms, occ, node_use, via_col, path_arena,
                               path_off, path_cap, path_len, dist_out, &W[t], tst + 16 * (int64_t)t);
             if (r < 0) {
 #pragma omp atomic write

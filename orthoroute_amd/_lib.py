@@ -60,7 +60,11 @@ class OrxNets(C.Structure):
     _fields_ = [
         ("n", C.c_int32), ("box", P), ("src", P), ("dst", P), ("path_arena", P),
         ("path_off", P), ("path_cap", P), ("path_len", P), ("path_cost", P), ("status", P),
+        ("seed_ptr", P), ("seed_node", P), ("seed_cost", P), ("tgt_ptr", P), ("tgt_node", P), ("tgt_cost", P),
     ]
+
+
+MAX_TARGETS = 128   # ORX_MAX_TARGETS
 
 
 class OrxWorkspace(C.Structure):

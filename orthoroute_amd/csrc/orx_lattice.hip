@@ -41,6 +41,12 @@ struct NetsDev {
     int32_t* path_len;
     float* path_cost;
     int32_t* status;
+    const int32_t* seed_ptr;
+    const int32_t* seed_node;
+    const float* seed_cost;
+    const int32_t* tgt_ptr;
+    const int32_t* tgt_node;
+    const float* tgt_cost;
 };
 
 struct Ctl {
@@ -52,6 +58,10 @@ struct Ctl {
     int32_t path_len;
     uint32_t rounds, phases, items;
     uint32_t fhist[32];   // far-pile histogram of log2(1 + (f - T) / delta)
+    unsigned long long tbest[2];   // best certified (total bits << 32 | node id), alternate checks
+    uint32_t tlb[2];               // bound of the uncertified targets' totals (fp32 bits)
+    uint32_t best_r;
+    float best_cost;
     unsigned long long settled, relaxed;
 };
 
@@ -60,6 +70,42 @@ struct Box {
     uint32_t wh, R;
     FastDiv dw, dwh;
 };
+
+// Terminal table of a multi-terminal net (find_path_multisource_multisink,
+// unified_pathfinder.py:2550-2659): the ROI indices of its targets inside the
+// box, ascending, and their ROI-local bounding box.  Targets are sinks: the
+// search reaches them but does not expand them (UP:2611-2618).
+constexpr int32_t kMaxTargets = ORX_MAX_TARGETS;
+struct TermTab {
+    uint32_t n;
+    uint32_t lx0, lx1, ly0, ly1, z0, z1;
+    uint32_t seeds_in;      // seeds inside the box
+    uint32_t roi[kMaxTargets];
+};
+
+// Seeds of a multi-terminal net: key fl(cost + q_seed), the seed paying its
+// own node term (cuda_dijkstra.py:5589-5606, UP:2582-2583).
+struct Multi {
+    const TermTab* tt;
+    const int32_t* seed_node;
+    const float* seed_cost;
+    int32_t ns;
+    const int32_t* tgt_node;
+    const float* tgt_cost;   // nullptr: all 0
+    int32_t nt;
+    Params P;
+};
+
+__device__ __forceinline__ bool is_target(const TermTab* T, uint32_t r, uint32_t lx, uint32_t ly, uint32_t z) {
+    if (lx < T->lx0 || lx > T->lx1 || ly < T->ly0 || ly > T->ly1 || z < T->z0 || z > T->z1) return false;
+    uint32_t lo = 0, hi = T->n;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (T->roi[m] < r) lo = m + 1;
+        else hi = m;
+    }
+    return lo < T->n && T->roi[lo] == r;
+}
 
 // Atomic helpers (workgroup scope: every access to an ROI's search state
 // comes from the one workgroup that owns the ROI).
@@ -112,6 +158,16 @@ __device__ __forceinline__ int64_t roi_global(const Lat& g, const Box& B, uint32
     return (int64_t)z * g.XY + (int64_t)(B.y0 + (int32_t)ly) * g.X + (B.x0 + (int32_t)lx);
 }
 
+// ROI-local index of a global node, UINT32_MAX outside the box.
+__device__ __forceinline__ uint32_t global_to_roi(const Lat& g, const Box& B, int64_t gid) {
+    const int32_t z = (int32_t)(gid / g.XY);
+    const int64_t rem = gid - (int64_t)z * g.XY;
+    const int32_t y = (int32_t)(rem / g.X);
+    const int32_t x = (int32_t)(rem - (int64_t)y * g.X);
+    if (x < B.x0 || x >= B.x0 + B.w || y < B.y0 || y >= B.y0 + B.h) return UINT32_MAX;
+    return ((uint32_t)z * (uint32_t)B.h + (uint32_t)(y - B.y0)) * (uint32_t)B.w + (uint32_t)(x - B.x0);
+}
+
 // ---------------------------------------------------------------------------
 // Accountant pieces run by one workgroup over one path.
 // ---------------------------------------------------------------------------
@@ -156,22 +212,25 @@ __device__ void reprice_path_block(const Lat& g, const Params& P, const int32_t*
     for (int32_t i = threadIdx.x; i < len; i += BLOCK) reprice_in_arcs(g, P, path[i]);
 }
 
-// A* heuristic toward the sink (global coords tx, ty, tz):
-//   h(v) = cp*(|x-tx| + |y-ty|) + cv*|z-tz|, evaluated exactly in double.
+// A* heuristic toward the sink box [tx0, tx1] x [ty0, ty1] x [tz0, tz1]
+// (global coords; one node for a two-pin net, the targets' bounding box for
+// a multi-terminal net):
+//   h(v) = cp*(dx + dy) + cv*dz, d* = distance to the box along that axis,
+// evaluated exactly in double.  Every sink has h = 0.
 // cp / cv are per-move lower bounds of the entering weight reduced by a
 // margin of at least half an fp32 ulp below gmax, so along every tight arc
 // u->v with g(v) < gmax, f(u) = g(u) + h(u) <= f(v): bucketing by f settles
-// exactly the canonical keys of the sink and its parent chain.  A result with
-// g(dst) >= gmax is recomputed without the heuristic by the caller.
+// exactly the canonical keys of the sinks and their parent chains.  A result
+// with a sink at g >= gmax is recomputed without the heuristic by the caller.
 struct Heur {
     bool on;
     double cp, cv;
-    int32_t tx, ty, tz;
+    int32_t tx0, tx1, ty0, ty1, tz0, tz1;
     __device__ __forceinline__ double h(int32_t x, int32_t y, int32_t z) const {
         if (!on) return 0.0;
-        const int32_t dx = x > tx ? x - tx : tx - x;
-        const int32_t dy = y > ty ? y - ty : ty - y;
-        const int32_t dz = z > tz ? z - tz : tz - z;
+        const int32_t dx = x < tx0 ? tx0 - x : (x > tx1 ? x - tx1 : 0);
+        const int32_t dy = y < ty0 ? ty0 - y : (y > ty1 ? y - ty1 : 0);
+        const int32_t dz = z < tz0 ? tz0 - z : (z > tz1 ? z - tz1 : 0);
         return cp * (double)(dx + dy) + cv * (double)dz;
     }
 };
@@ -213,13 +272,68 @@ __device__ __forceinline__ void ring_put(uint32_t* lq, uint32_t* gq, uint32_t i,
     else gq[i] = v;
 }
 
+// Seeds of a multi-terminal net: every seed inside the box gets the key
+// (fl(cost + q_seed), 0 hops, root), duplicates keeping the minimum; the
+// first threshold is delta above the smallest seed f, and the seeds below
+// it form round 0's near queue, the others the far pile.
+template <int BLOCK>
+__device__ double seed_init(const Lat& g, const Box& B, const Multi& M, float delta, const Heur& H, uint64_t* keys,
+                            uint32_t* bmn0, uint32_t* bmf, uint32_t* qA, uint32_t* lqA, uint32_t* fA, Ctl* ctl) {
+    const uint32_t tid = threadIdx.x;
+    for (int32_t i = (int32_t)tid; i < M.ns; i += BLOCK) {
+        const int64_t gs = M.seed_node[i];
+        const uint32_t rs = global_to_roi(g, B, gs);
+        if (rs == UINT32_MAX) continue;
+        const float q = node_cost(g.node_hist[gs],
+                                  __hip_atomic_load(g.node_use + gs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), M.P);
+        key_min(keys + rs, kpack(__fadd_rn(M.seed_cost[i], q), 0u, ORX_ROOT_SLOT));
+    }
+    __syncthreads();
+    double lmin = INFINITY;
+    for (int32_t i = (int32_t)tid; i < M.ns; i += BLOCK) {
+        const uint32_t rs = global_to_roi(g, B, M.seed_node[i]);
+        if (rs == UINT32_MAX) continue;
+        uint32_t lx, ly, z;
+        roi_decode(B, rs, lx, ly, z);
+        lmin = fmin(lmin, (double)kdist(key_load(keys + rs)) + box_h(H, B, lx, ly, z));
+    }
+    for (int o = 32; o > 0; o >>= 1) lmin = fmin(lmin, __shfl_xor(lmin, o));
+    if (lane_id() == 0 && lmin < (double)INFINITY) atomicMin(&ctl->minf, dbits(lmin));
+    __syncthreads();
+    const double f0 = __longlong_as_double((long long)ctl->minf);
+    double T = f0 + (double)delta;
+    if (!(T > f0)) T = nextafter(f0, (double)INFINITY);
+    for (int32_t base = 0; base < M.ns; base += BLOCK) {
+        const int32_t i = base + (int32_t)tid;
+        bool near = false, far = false;
+        uint32_t rs = UINT32_MAX;
+        if (i < M.ns) rs = global_to_roi(g, B, M.seed_node[i]);
+        if (rs != UINT32_MAX) {
+            uint32_t lx, ly, z;
+            roi_decode(B, rs, lx, ly, z);
+            const double f = (double)kdist(key_load(keys + rs)) + box_h(H, B, lx, ly, z);
+            if (f < T) near = bit_set(bmn0, rs) == 0;
+            else far = bit_set(bmf, rs) == 0;
+        }
+        int at = wave_append(near, &ctl->qcnt[0]);
+        if (near) ring_put(lqA, qA, (uint32_t)at, rs);
+        at = wave_append(far, &ctl->fcnt);
+        if (far) fA[at] = rs;
+    }
+    __syncthreads();
+    return T;
+}
+
 // skip_keys / skip_bitmaps: those arrays were cleared before the launch
 // (global workspace ranges filled chip-wide by ws_fill_kernel).
+// multi == false: one source rsrc at key 0 and one sink rdst.  Otherwise the
+// seeds and targets of M (rsrc / rdst unused); the search then stops once
+// the best certified target cannot be beaten (see the stop test below).
 template <int BLOCK>
 __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rdst, bool full, float delta,
                          const Heur& H, uint64_t* keys, uint32_t* bmn0, uint32_t* bmn1, uint32_t* bmf, uint32_t* qA,
                          uint32_t* qB, uint32_t* lqA, uint32_t* lqB, uint32_t* fA, uint32_t* fB, bool skip_keys,
-                         bool skip_bitmaps, Ctl* ctl) {
+                         bool skip_bitmaps, Ctl* ctl, bool multi, const Multi& M) {
     const uint32_t tid = threadIdx.x;
     const uint32_t R = B.R;
     const uint32_t nwords = (R + 31) / 32;
@@ -228,26 +342,32 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
     if (!skip_bitmaps)
         for (uint32_t i = tid; i < nwords; i += BLOCK) { bmn0[i] = 0; bmn1[i] = 0; bmf[i] = 0; }
     if (tid == 0) {
-        ctl->qcnt[0] = 1; ctl->qcnt[1] = 0; ctl->qcnt[2] = 0;
+        ctl->qcnt[0] = multi ? 0u : 1u; ctl->qcnt[1] = 0; ctl->qcnt[2] = 0;
         ctl->fcnt = 0; ctl->fcnt2 = 0; ctl->err = 0;
+        ctl->tbest[0] = ctl->tbest[1] = ~0ull;
+        ctl->tlb[0] = ctl->tlb[1] = 0x7F800000u;
+        ctl->minf = dbits((double)INFINITY);
     }
     __syncthreads();
     double T;
-    {
+    if (multi) {
+        T = seed_init<BLOCK>(g, B, M, delta, H, keys, bmn0, bmf, qA, lqA, fA, ctl);
+    } else {
         uint32_t lx, ly, z;
         roi_decode(B, rsrc, lx, ly, z);
         const double f0 = box_h(H, B, lx, ly, z);
         T = f0 + (double)delta;
         if (!(T > f0)) T = nextafter(f0, (double)INFINITY);
+        if (tid == 0) {
+            keys[rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
+            lqA[0] = rsrc;
+            bit_set(bmn0, rsrc);
+        }
+        __syncthreads();
     }
-    if (tid == 0) {
-        keys[rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
-        lqA[0] = rsrc;
-        bit_set(bmn0, rsrc);
-    }
-    __syncthreads();
 
-    uint32_t r = 0;   // round counter
+    uint32_t r = 0;     // round counter
+    uint32_t chk = 0;   // target checks (multi-terminal stop test)
     for (;;) {
         // ---------------- near phase: relax until the bucket [.., T) drains
         for (;;) {
@@ -279,11 +399,13 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
                     ku = key_load(keys + u);
                     roi_decode(B, u, lx, ly, z);
                     horiz = (g.hmask >> z) & 1u;
-                    const int64_t gu = roi_global(g, B, lx, ly, z);
-                    const float4 wv = reinterpret_cast<const float4*>(g.wcost)[gu];
-                    w4[0] = wv.x; w4[1] = wv.y; w4[2] = wv.z; w4[3] = wv.w;
+                    if (!(multi && is_target(M.tt, u, lx, ly, z))) {   // targets are sinks
+                        const int64_t gu = roi_global(g, B, lx, ly, z);
+                        const float4 wv = reinterpret_cast<const float4*>(g.wcost)[gu];
+                        w4[0] = wv.x; w4[1] = wv.y; w4[2] = wv.z; w4[3] = wv.w;
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) nb[s] = roi_nbr(B, horiz, u, lx, ly, z, s);
+                        for (int s = 0; s < 4; ++s) nb[s] = roi_nbr(B, horiz, u, lx, ly, z, s);
+                    }
                 }
                 const float du = kdist(ku);
                 const uint32_t hu = khops(ku);
@@ -332,7 +454,41 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
             __syncthreads();
         }
         // ---------------- every key with f below T is final
-        if (!full) {
+        if (!full && multi) {
+            // A target with dist < T' = min(T, gmax) (T alone without the
+            // heuristic) is certified: final and exact (h = 0 on targets).
+            // Every other target has a true dist >= T', so its total is at
+            // least fl(T'^ + cost), T'^ the smallest fp32 >= T'.  Stop when
+            // that bound exceeds the best certified total for all of them.
+            // Checks alternate between two slot pairs: the pair read now is
+            // reset after the next check's barrier, when every lane has read it.
+            const uint32_t par = chk++ & 1u;
+            double Tc = T;
+            if (H.on && Tc > (double)M.P.astar_gmax) Tc = (double)M.P.astar_gmax;
+            float Tu = (float)Tc;
+            if ((double)Tu < Tc) Tu = nextafterf(Tu, INFINITY);
+            if ((int32_t)tid < M.nt) {
+                const int64_t gid = M.tgt_node[tid];
+                const uint32_t rt = global_to_roi(g, B, gid);
+                if (rt != UINT32_MAX) {
+                    const uint64_t kt = key_load(keys + rt);
+                    const float c = M.tgt_cost ? M.tgt_cost[tid] : 0.0f;
+                    if (kt != ORX_KEY_INF && (double)kdist(kt) < Tc) {
+                        const float tot = __fadd_rn(kdist(kt), c);
+                        if (tot < INFINITY)
+                            atomicMin(&ctl->tbest[par],
+                                      ((unsigned long long)__float_as_uint(tot) << 32) | (uint32_t)gid);
+                    } else {
+                        atomicMin(&ctl->tlb[par], __float_as_uint(__fadd_rn(Tu, c)));
+                    }
+                }
+            }
+            __syncthreads();
+            const unsigned long long bb = ctl->tbest[par];
+            const uint32_t lb = ctl->tlb[par];
+            if (tid == 0) { ctl->tbest[par ^ 1u] = ~0ull; ctl->tlb[par ^ 1u] = 0x7F800000u; }
+            if (bb != ~0ull && lb > (uint32_t)(bb >> 32)) break;
+        } else if (!full) {
             const uint64_t kd = key_load(keys + rdst);
             if (kd != ORX_KEY_INF && (double)kdist(kd) < T) break;  // h(dst) = 0
         }
@@ -427,9 +583,11 @@ __device__ void box_sssp(const Lat& g, const Box& B, uint32_t rsrc, uint32_t rds
 // sink (every reached node without a sink), and the in-ROI out-degree of
 // those popped before the sink.  ROI-local indices order like global ids.
 // Every node ordered before the sink has f <= f(dst) < T, so its key is final.
+// With a terminal table, rdst is the chosen target (full: none chosen) and
+// no target contributes out-arcs (targets are sinks).
 template <int BLOCK>
 __device__ void box_stats(const Lat& g, const Box& B, const Heur& H, const uint64_t* keys, uint32_t rdst, bool full,
-                          Ctl* ctl) {
+                          Ctl* ctl, const TermTab* TT = nullptr) {
     const uint64_t kd = full ? ORX_KEY_INF : key_load(keys + rdst);
     const double fd = full ? (double)INFINITY : (double)kdist(kd);
     unsigned long long settled = 0, relaxed = 0;
@@ -443,7 +601,7 @@ __device__ void box_stats(const Lat& g, const Box& B, const Heur& H, const uint6
             if (f > fd || (f == fd && (k > kd || (k == kd && r > rdst)))) continue;
         }
         settled += 1;
-        if (!full && r == rdst) continue;
+        if (TT ? is_target(TT, r, lx, ly, z) : (!full && r == rdst)) continue;
         const bool horiz = (g.hmask >> z) & 1u;
         for (int s = 0; s < 4; ++s) relaxed += roi_nbr(B, horiz, r, lx, ly, z, s) != UINT32_MAX;
     }
@@ -489,13 +647,15 @@ __device__ __constant__ int8_t kStencilLane[343] = {
 // up to four parent links through shuffles (the nodes it looks up are at
 // most three moves away; the same walk on every lane, so control flow stays
 // uniform).  One dependent memory round trip per four hops instead of one
-// per hop.  Lane 0 writes the path, root first.
+// per hop.  Lane 0 writes the path, root first.  The walk must end at rsrc,
+// or at any root (a seed) when rsrc is UINT32_MAX.  Returns the path length,
+// 0 when rdst was not reached, or a negative status.
 // Call with all 64 lanes of one wave; every lane returns the same value.
 __device__ int32_t box_backtrace_wave(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rsrc,
                                       uint32_t rdst, int32_t* path, int32_t cap) {
     const uint32_t lane = lane_id();
     const uint64_t kd = key_load(keys + rdst);
-    if (kd == ORX_KEY_INF) return ORX_NO_PATH;
+    if (kd == ORX_KEY_INF) return 0;
     const uint32_t hops = khops(kd);
     if ((int64_t)hops + 1 > cap) return ORX_E_OVERFLOW;
     const int32_t sdx = lane < 63 ? kStencilDx[lane] : 0;
@@ -518,7 +678,7 @@ __device__ int32_t box_backtrace_wave(const Lat& g, const Box& B, const uint64_t
             const uint32_t v = ((uint32_t)vz * (uint32_t)B.h + (uint32_t)vy) * (uint32_t)B.w + (uint32_t)vx;
             if (lane == 0) path[i] = (int32_t)roi_global(g, B, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
             const uint32_t sl = kslot(k);
-            if (i == 0) return (sl == ORX_ROOT_SLOT && v == rsrc) ? (int32_t)hops + 1 : ORX_E_CYCLE;
+            if (i == 0) return (sl == ORX_ROOT_SLOT && (rsrc == UINT32_MAX || v == rsrc)) ? (int32_t)hops + 1 : ORX_E_CYCLE;
             if (sl > 3) return ORX_E_CYCLE;
             int32_t mx = 0, my = 0, mz = 0;
             if (sl == 0) {
@@ -544,10 +704,22 @@ __device__ __forceinline__ Heur make_heur(const Lat& g, const Params& P, int64_t
     H.on = allowed && P.astar_cp > 0.0f && P.astar_cv > 0.0f && P.astar_gmax > 0.0f;
     H.cp = P.astar_cp;
     H.cv = P.astar_cv;
-    H.tz = (int32_t)(dst / g.XY);
-    const int64_t rem = dst - (int64_t)H.tz * g.XY;
-    H.ty = (int32_t)(rem / g.X);
-    H.tx = (int32_t)(rem - (int64_t)H.ty * g.X);
+    H.tz0 = H.tz1 = (int32_t)(dst / g.XY);
+    const int64_t rem = dst - (int64_t)H.tz0 * g.XY;
+    H.ty0 = H.ty1 = (int32_t)(rem / g.X);
+    H.tx0 = H.tx1 = (int32_t)(rem - (int64_t)H.ty0 * g.X);
+    return H;
+}
+
+// Heuristic toward the bounding box of a terminal table's targets.
+__device__ __forceinline__ Heur make_heur_box(const Params& P, const Box& B, const TermTab& T, bool allowed) {
+    Heur H;
+    H.on = allowed && P.astar_cp > 0.0f && P.astar_cv > 0.0f && P.astar_gmax > 0.0f;
+    H.cp = P.astar_cp;
+    H.cv = P.astar_cv;
+    H.tx0 = B.x0 + (int32_t)T.lx0; H.tx1 = B.x0 + (int32_t)T.lx1;
+    H.ty0 = B.y0 + (int32_t)T.ly0; H.ty1 = B.y0 + (int32_t)T.ly1;
+    H.tz0 = (int32_t)T.z0; H.tz1 = (int32_t)T.z1;
     return H;
 }
 
@@ -559,15 +731,6 @@ __device__ __forceinline__ Box make_box(const int32_t* b, int32_t L) {
     B.dw = FastDiv((uint32_t)B.w);
     B.dwh = FastDiv(B.wh);
     return B;
-}
-
-__device__ __forceinline__ uint32_t global_to_roi(const Lat& g, const Box& B, int64_t gid) {
-    const int32_t z = (int32_t)(gid / g.XY);
-    const int64_t rem = gid - (int64_t)z * g.XY;
-    const int32_t y = (int32_t)(rem / g.X);
-    const int32_t x = (int32_t)(rem - (int64_t)y * g.X);
-    if (x < B.x0 || x >= B.x0 + B.w || y < B.y0 || y >= B.y0 + B.h) return UINT32_MAX;
-    return ((uint32_t)z * (uint32_t)B.h + (uint32_t)(y - B.y0)) * (uint32_t)B.w + (uint32_t)(x - B.x0);
 }
 
 // Search-state pointers of one wave slot.  The dynamic LDS of a search
@@ -619,12 +782,95 @@ __device__ __forceinline__ SlotState slot_state(const orx_workspace& ws, int64_t
     return S;
 }
 
+// Terminal table of one net: its targets inside the box (sorted ROI indices
+// and their bounding box) and the number of its seeds inside the box.
+template <int BLOCK>
+__device__ void build_terms(const Lat& g, const Box& B, const int32_t* tn, int32_t nt, const int32_t* sn, int32_t ns,
+                            TermTab& T) {
+    static_assert(kMaxTargets <= BLOCK, "one lane per target");
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        T.n = 0; T.seeds_in = 0;
+        T.lx0 = T.ly0 = T.z0 = UINT32_MAX;
+        T.lx1 = T.ly1 = T.z1 = 0;
+    }
+    __syncthreads();
+    if ((int32_t)tid < nt) {
+        const uint32_t r = global_to_roi(g, B, tn[tid]);
+        if (r != UINT32_MAX) {
+            uint32_t lx, ly, z;
+            roi_decode(B, r, lx, ly, z);
+            T.roi[atomicAdd(&T.n, 1u)] = r;
+            atomicMin(&T.lx0, lx); atomicMax(&T.lx1, lx);
+            atomicMin(&T.ly0, ly); atomicMax(&T.ly1, ly);
+            atomicMin(&T.z0, z); atomicMax(&T.z1, z);
+        }
+    }
+    uint32_t sin = 0;
+    for (int32_t i = (int32_t)tid; i < ns; i += BLOCK) sin += global_to_roi(g, B, sn[i]) != UINT32_MAX ? 1u : 0u;
+    sin = wave_sum(sin);
+    if (lane_id() == 0 && sin) atomicAdd(&T.seeds_in, sin);
+    __syncthreads();
+    // ascending order: each entry's rank (ties by position)
+    const uint32_t n = T.n;
+    uint32_t v = 0, rank = 0;
+    if (tid < n) {
+        v = T.roi[tid];
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t o = T.roi[j];
+            rank += (o < v || (o == v && j < tid)) ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    if (tid < n) T.roi[rank] = v;
+    __syncthreads();
+}
+
+// Best target of a finished multi-terminal search, by one wavefront: the
+// reached target with the smallest (fl(dist + target cost), node id) and a
+// finite total (UP:2605-2618 keeps the first strict improvement in pop
+// order).  Targets the search did not certify have totals above the
+// certified best, so they never win.
+__device__ void pick_target(const Lat& g, const Box& B, const uint64_t* keys, const int32_t* tn, const float* tc,
+                            int32_t nt, Ctl& ctl) {
+    const uint32_t lane = lane_id();
+    float bt = INFINITY;
+    int64_t bg = INT64_MAX;
+    for (int32_t i = (int32_t)lane; i < nt; i += 64) {
+        const int64_t gid = tn[i];
+        const uint32_t r = global_to_roi(g, B, gid);
+        if (r == UINT32_MAX) continue;
+        const uint64_t k = key_load(keys + r);
+        if (k == ORX_KEY_INF) continue;
+        const float t = __fadd_rn(kdist(k), tc ? tc[i] : 0.0f);
+        if (!(t < INFINITY)) continue;
+        if (t < bt || (t == bt && gid < bg)) { bt = t; bg = gid; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ot = __shfl_xor(bt, o);
+        const int64_t og = __shfl_xor(bg, o);
+        if (ot < bt || (ot == bt && og < bg)) { bt = ot; bg = og; }
+    }
+    if (lane == 0) {
+        ctl.best_r = bg == INT64_MAX ? UINT32_MAX : global_to_roi(g, B, bg);
+        ctl.best_cost = bt;
+    }
+}
+
 // The per-net body of _route_all (unified_pathfinder.py:7949-8430), run by
 // one workgroup: rip-up, search, backtrace, commit.  Ends with a barrier.
+//
+// A net with seeds in nets.seed_ptr is a multi-terminal net
+// (find_path_multisource_multisink, UP:2550-2659, on the net's box): seeds
+// priced with their cost plus their node term, targets not expanded, the
+// best target by (fl(dist + target cost), node id); no path when no seed or
+// target lies in the box, nothing is reached, or the best target is itself a
+// seed (a one-node path, UP:2647-2648).  Other nets route src -> dst.
+// Both forms share one search call site (one inlined copy of box_sssp).
 template <int BLOCK>
 __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, int32_t k, const orx_workspace& ws,
                           int64_t ws_off, float delta, int32_t lds_nodes, int32_t lds_region, int32_t flags,
-                          unsigned long long* stats, uint32_t* lds, Ctl& ctl) {
+                          unsigned long long* stats, uint32_t* lds, Ctl& ctl, TermTab& TT) {
     const Box B = make_box(nets.box + 4 * (int64_t)k, g.L);
     const SlotState S = slot_state(ws, ws_off, B.R, lds, lds_region, lds_nodes);
     int32_t* path = nets.path_arena + nets.path_off[k];
@@ -639,47 +885,95 @@ __device__ void route_net(const Lat& g, const Params& P, const NetsDev& nets, in
         reprice_path_block<BLOCK>(g, P, path, old_len);
         __syncthreads();
     }
-    // 2. search
-    const uint32_t rsrc = global_to_roi(g, B, nets.src[k]);
-    const uint32_t rdst = global_to_roi(g, B, nets.dst[k]);
-    int32_t len = 0;
-    if (rsrc == UINT32_MAX || rdst == UINT32_MAX) {
-        if (threadIdx.x == 0) { nets.status[k] = ORX_E_ARG; nets.path_len[k] = 0; }
-        len = ORX_E_ARG;
-    } else {
-        const bool full = (flags & ORX_F_FULL) != 0;
-        Heur H = make_heur(g, P, nets.dst[k], !full);
-        const bool pre = (flags & kPrefilled) != 0;
-        box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB,
-                        S.fA, S.fB, pre && S.keys_global, pre && S.bm_global, &ctl);
-        if (H.on) {
-            const uint64_t kd = key_load(S.keys + rdst);
-            if (kd != ORX_KEY_INF && !(kdist(kd) < P.astar_gmax)) {
-                // distances this large may round below the A* margin: redo plainly
-                H.on = false;
-                box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA,
-                                S.lqB, S.fA, S.fB, false, false, &ctl);
-            }
+    // 2. terminals
+    const bool full = (flags & ORX_F_FULL) != 0;
+    const bool multi = nets.seed_ptr && nets.seed_ptr[k + 1] > nets.seed_ptr[k];
+    int32_t st = ORX_OK;
+    uint32_t rsrc = 0, rdst = 0;
+    const int32_t* tn = nullptr;
+    const float* tc = nullptr;
+    int32_t nt = 0;
+    Multi M;
+    Heur H;
+    if (multi) {
+        const int32_t s0 = nets.seed_ptr[k];
+        const int32_t t0 = nets.tgt_ptr ? nets.tgt_ptr[k] : 0;
+        nt = nets.tgt_ptr ? nets.tgt_ptr[k + 1] - t0 : 0;
+        if (nt < 1 || nt > kMaxTargets || !nets.seed_node || !nets.seed_cost || !nets.tgt_node) {
+            st = ORX_E_ARG;
+        } else {
+            tn = nets.tgt_node + t0;
+            tc = nets.tgt_cost ? nets.tgt_cost + t0 : nullptr;
+            M.tt = &TT; M.seed_node = nets.seed_node + s0; M.seed_cost = nets.seed_cost + s0;
+            M.ns = nets.seed_ptr[k + 1] - s0; M.P = P;
+            M.tgt_node = tn; M.tgt_cost = tc; M.nt = nt;
+            build_terms<BLOCK>(g, B, tn, nt, M.seed_node, M.ns, TT);
+            if (TT.n == 0 || TT.seeds_in == 0) st = ORX_NO_PATH;
+            H = make_heur_box(P, B, TT, !full);
         }
-        if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
-        // 3. backtrace
-        if (threadIdx.x < 64) {
-            const int32_t l = box_backtrace_wave(g, B, S.keys, rsrc, rdst, path, cap);
-            if (threadIdx.x == 0) {
-                ctl.path_len = l;
-                nets.status[k] = l > 0 ? ORX_OK : l;
-                nets.path_len[k] = l > 0 ? l : 0;
-                nets.path_cost[k] = l > 0 ? kdist(key_load(S.keys + rdst)) : INFINITY;
+        rsrc = UINT32_MAX;   // the walk may end at any seed
+    } else {
+        rsrc = global_to_roi(g, B, nets.src[k]);
+        rdst = global_to_roi(g, B, nets.dst[k]);
+        if (rsrc == UINT32_MAX || rdst == UINT32_MAX) st = ORX_E_ARG;
+        H = make_heur(g, P, nets.dst[k], !full);
+    }
+    int32_t len = 0;
+    float cost = INFINITY;
+    if (st == ORX_OK) {
+        // 3. search; redone without the heuristic when the sink's distance
+        // (a multi-terminal net: the chosen total) is >= gmax, where the A*
+        // margin no longer covers the rounding.  Below it the result is
+        // exact: the true best total is no larger, so the true best target
+        // lies below gmax and is certified.
+        bool pre = (flags & kPrefilled) != 0;
+        for (;;) {
+            box_sssp<BLOCK>(g, B, multi ? 0u : rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA,
+                            S.qB, S.lqA, S.lqB, S.fA, S.fB, pre && S.keys_global, pre && S.bm_global, &ctl,
+                            multi, M);
+            bool big;
+            if (multi) {
+                if (threadIdx.x < 64) pick_target(g, B, S.keys, tn, tc, nt, ctl);
+                __syncthreads();
+                big = ctl.best_r != UINT32_MAX && !(ctl.best_cost < P.astar_gmax);
+            } else {
+                const uint64_t kd = key_load(S.keys + rdst);
+                big = kd != ORX_KEY_INF && !(kdist(kd) < P.astar_gmax);
             }
+            if (!H.on || !big) break;
+            H.on = false;
+            pre = false;
+        }
+        bool all = full;
+        if (multi) {
+            rdst = ctl.best_r;
+            all = full || rdst == UINT32_MAX;
+        }
+        if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, all, &ctl, multi ? &TT : nullptr);
+        // 4. backtrace
+        if (rdst != UINT32_MAX && threadIdx.x < 64) {
+            const int32_t l = box_backtrace_wave(g, B, S.keys, rsrc, rdst, path, cap);
+            if (threadIdx.x == 0) ctl.path_len = l;
         }
         __syncthreads();
-        len = ctl.path_len;
-        // 4. commit and re-price
-        if (len > 0) {
-            apply_path_block<BLOCK>(g, path, len, +1);
-            __syncthreads();
-            reprice_path_block<BLOCK>(g, P, path, len);
+        const int32_t l = rdst == UINT32_MAX ? 0 : ctl.path_len;
+        if (l < 0) st = l;
+        else if (l == 0 || (multi && l == 1)) st = ORX_NO_PATH;
+        else {
+            len = l;
+            cost = multi ? ctl.best_cost : kdist(key_load(S.keys + rdst));
         }
+    }
+    if (threadIdx.x == 0) {
+        nets.status[k] = st;
+        nets.path_len[k] = len;
+        nets.path_cost[k] = cost;
+    }
+    // 5. commit and re-price
+    if (len > 0) {
+        apply_path_block<BLOCK>(g, path, len, +1);
+        __syncthreads();
+        reprice_path_block<BLOCK>(g, P, path, len);
     }
     if (stats && threadIdx.x == 0) {
         atomicAdd(stats + 0, ctl.settled);
@@ -700,8 +994,9 @@ __global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, Nets
                                                            unsigned long long* stats) {
     ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(Ctl, ctl);
+    ORX_SHARED(TermTab, tt);
     route_net<BLOCK>(g, P, nets, wave_nets[blockIdx.x], ws, ws_off[blockIdx.x], delta, lds_nodes, lds_region, flags,
-                     stats, reinterpret_cast<uint32_t*>(lds_mem), ctl);
+                     stats, reinterpret_cast<uint32_t*>(lds_mem), ctl, tt);
 }
 
 // Persistent dependency-driven form: workgroups claim nets in schedule order
@@ -718,6 +1013,7 @@ __global__ __launch_bounds__(BLOCK) void route_dag_kernel(Lat g, Params P, NetsD
                                                           unsigned long long* stats) {
     ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(Ctl, ctl);
+    ORX_SHARED(TermTab, tt);
     ORX_SHARED(int32_t, s_pick);
     for (;;) {
         if (threadIdx.x == 0) s_pick = (int32_t)atomicAdd(next, 1u);
@@ -734,7 +1030,7 @@ __global__ __launch_bounds__(BLOCK) void route_dag_kernel(Lat g, Params P, NetsD
         }
         __syncthreads();
         route_net<BLOCK>(g, P, nets, k, ws, ws_off[k], delta, lds_nodes, lds_region, flags, stats,
-                         reinterpret_cast<uint32_t*>(lds_mem), ctl);
+                         reinterpret_cast<uint32_t*>(lds_mem), ctl, tt);
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_store(done + k, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -787,8 +1083,9 @@ __global__ __launch_bounds__(BLOCK) void sssp_box_kernel(Lat g, orx_workspace ws
     }
     Heur H;
     H.on = false;
+    const Multi none{};
     box_sssp<BLOCK>(g, B, rsrc, rdst, full, delta, H, S.keys, S.bmn0, S.bmn1, S.bmf, S.qA, S.qB, S.lqA, S.lqB, S.fA,
-                    S.fB, false, false, &ctl);
+                    S.fB, false, false, &ctl, false, none);
     if (flags & ORX_F_STATS) box_stats<BLOCK>(g, B, H, S.keys, rdst, full, &ctl);
     for (uint32_t r = threadIdx.x; r < B.R; r += BLOCK) {
         const uint64_t k = key_load(S.keys + r);
@@ -1432,7 +1729,8 @@ int launch_route_wave(const Lat& g, const Params& P, const NetsDev& nd, const in
 
 NetsDev make_nets(const orx_nets* nets) {
     return NetsDev{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
-                   nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+                   nets->path_cap, nets->path_len, nets->path_cost, nets->status, nets->seed_ptr,
+                   nets->seed_node, nets->seed_cost, nets->tgt_ptr, nets->tgt_node, nets->tgt_cost};
 }
 
 }  // namespace
@@ -1545,8 +1843,7 @@ int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* n
     if (ws->ws_nodes % 64) return orx_host::fail(ORX_E_ARG, "ws_nodes must be a multiple of 64");
     const LdsPlan lp = lds_plan(max_nodes, cfg->lds_nodes);
     const size_t lds = lp.bytes;
-    NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
-               nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+    const NetsDev nd = make_nets(nets);
     unsigned long long* st = (unsigned long long*)cfg->stats;
     const hipStream_t s = (hipStream_t)stream;
     const Lat g = make_lat(lat);
@@ -1585,8 +1882,7 @@ int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_c
     if (rc) return rc;
     if (!nets || !d_over || !d_hops) return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (nets->n <= 0) return ORX_OK;
-    NetsDev nd{nets->n, nets->box, nets->src, nets->dst, nets->path_arena, nets->path_off,
-               nets->path_cap, nets->path_len, nets->path_cost, nets->status};
+    const NetsDev nd = make_nets(nets);
     hipLaunchKernelGGL(net_overuse_kernel, dim3((nets->n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                        make_lat(lat), nd, edge_cap, d_over, d_hops);
     ORX_HIP_CHECK(hipGetLastError());

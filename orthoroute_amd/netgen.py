@@ -13,7 +13,10 @@ layers.  Workloads:
 * ``uneven_800`` -- 800x800x20, 8000 two-pin nets, Manhattan length
   lognormal (median 25, sigma 1, clipped to 2..700), sources uniform,
   directions random, seed 3, ROI margin 4;
-* ``serving_job`` -- one synthetic board of the serving stream.
+* ``serving_job`` -- one synthetic board of the serving stream;
+* ``with_portals`` -- any of the above with portal terminals: every pad
+  becomes a seed (source pad) or target (sink pad) on each inner layer of
+  one or more escape cells, priced by via depth.
 """
 from __future__ import annotations
 
@@ -24,6 +27,64 @@ import numpy as np
 from .lattice import LatticeSpec
 
 
+MAX_TARGETS = 128   # targets per net the route kernel accepts (ORX_MAX_TARGETS)
+
+
+@dataclass
+class Terminals:
+    """Seeds and targets of multi-terminal nets, CSR by net: net k's seeds
+    are seed_node[seed_ptr[k]:seed_ptr[k+1]] with costs seed_cost[...], its
+    targets likewise.  A net with no seeds routes src -> dst.  The search
+    semantics are those of find_path_multisource_multisink
+    (unified_pathfinder.py:2550-2659): each seed starts at fl(seed_cost +
+    its node term), targets are sinks, the best target minimises
+    (fl(dist + tgt_cost), node id).  Seeds and targets of a pad on every
+    legal layer are what _get_pad_portal_seeds (UP:4054-4180) lists."""
+    seed_ptr: np.ndarray   # int32[n+1]
+    seed_node: np.ndarray  # int32 global ids
+    seed_cost: np.ndarray  # float32
+    tgt_ptr: np.ndarray    # int32[n+1]
+    tgt_node: np.ndarray
+    tgt_cost: np.ndarray
+
+    def select(self, idx) -> "Terminals":
+        idx = np.asarray(idx, np.int64)
+
+        def take(ptr, *arrs):
+            lo, hi = ptr[idx], ptr[idx + 1]
+            cnt = (hi - lo).astype(np.int64)
+            nptr = np.zeros(len(idx) + 1, np.int32)
+            np.cumsum(cnt, out=nptr[1:])
+            sel = np.concatenate([np.arange(a, b) for a, b in zip(lo, hi)]) if len(idx) else np.zeros(0, np.int64)
+            sel = sel.astype(np.int64)
+            return (nptr, *(a[sel].copy() for a in arrs))
+
+        sp, sn, sc = take(self.seed_ptr, self.seed_node, self.seed_cost)
+        tp, tn, tc = take(self.tgt_ptr, self.tgt_node, self.tgt_cost)
+        return Terminals(sp, sn, sc, tp, tn, tc)
+
+    def counts(self) -> tuple[np.ndarray, np.ndarray]:
+        return np.diff(self.seed_ptr), np.diff(self.tgt_ptr)
+
+    def validate(self, n: int, N: int) -> None:
+        for name, ptr, node in (("seed", self.seed_ptr, self.seed_node), ("target", self.tgt_ptr, self.tgt_node)):
+            if len(ptr) != n + 1 or ptr[0] != 0 or np.any(np.diff(ptr) < 0) or ptr[-1] != len(node):
+                raise ValueError(f"{name}_ptr is not a CSR index over {n} nets")
+            if len(node) and (node.min() < 0 or node.max() >= N):
+                raise ValueError(f"{name} node outside the lattice")
+        ns, nt = self.counts()
+        multi = ns > 0
+        if np.any(multi & ((nt < 1) | (nt > MAX_TARGETS))):
+            raise ValueError(f"a multi-terminal net needs 1..{MAX_TARGETS} targets")
+        if len(self.seed_cost) != len(self.seed_node) or len(self.tgt_cost) != len(self.tgt_node):
+            raise ValueError("terminal costs and nodes differ in length")
+        # packed keys order distances by their fp32 bits: costs must be >= 0
+        if np.any(~np.isfinite(self.seed_cost)) or np.any(self.seed_cost < 0):
+            raise ValueError("seed costs must be finite and >= 0")
+        if np.any(np.isnan(self.tgt_cost)) or np.any(self.tgt_cost < 0):
+            raise ValueError("target costs must be >= 0")
+
+
 @dataclass
 class Netlist:
     spec: LatticeSpec
@@ -32,6 +93,7 @@ class Netlist:
     box: np.ndarray   # int32 [n, 4]: x0, y0, w, h
     name: str = ""
     meta: dict = field(default_factory=dict)
+    terms: Terminals | None = None
 
     @property
     def n(self) -> int:
@@ -41,14 +103,14 @@ class Netlist:
         return self.box[:, 2].astype(np.int64) * self.box[:, 3] * self.spec.L
 
     def subset(self, k: int) -> "Netlist":
-        return Netlist(self.spec, self.src[:k].copy(), self.dst[:k].copy(), self.box[:k].copy(),
-                       f"{self.name}[:{k}]", dict(self.meta))
+        return self.select(np.arange(min(k, self.n)), f":{k}")
 
     def select(self, idx, tag: str = "") -> "Netlist":
         """The nets ``idx`` (in that order) on the same board."""
         idx = np.asarray(idx, np.int64)
         return Netlist(self.spec, self.src[idx].copy(), self.dst[idx].copy(), self.box[idx].copy(),
-                       f"{self.name}[{tag or len(idx)}]", dict(self.meta))
+                       f"{self.name}[{tag or len(idx)}]", dict(self.meta),
+                       None if self.terms is None else self.terms.select(idx))
 
 
 def roi_boxes(spec: LatticeSpec, src: np.ndarray, dst: np.ndarray, margin: int) -> np.ndarray:
@@ -167,3 +229,64 @@ def serving_job(rng, n_min: int = 500, n_max: int = 10000, margin: int = 4) -> N
     n = min(n, (side * side) // 4)
     seed = int(rng.integers(0, 2**31 - 1))
     return uneven(side, side, L, n, seed, margin, f"job_{side}x{side}x{L}_{n}")
+
+
+def with_portals(nl: Netlist, candidates: int = 1, via_cost: float = 3.0, discount: float = 0.15,
+                 candidate_penalty: float = 0.5, fraction: float = 1.0, seed: int = 0) -> Netlist:
+    """Portal form of a netlist (the pads of _get_pad_portal_seeds,
+    unified_pathfinder.py:3510-3565 and 4054-4180): each pad at (x, y) on
+    layer 0 gets ``candidates`` escape cells (the pad cell, then seeded cells
+    within 2 of it), and each escape cell one terminal per inner layer
+    1..L-2 priced fp32(via_cost * discount * layer + candidate_penalty * c)
+    for the c-th cell.  Source pads give seeds, sink pads targets.  A seeded
+    ``fraction`` of the nets gets portals; the rest stay two-pin.  Boxes grow
+    to hold every terminal."""
+    spec = nl.spec
+    if spec.L < 3:
+        raise ValueError("portals need at least three layers")
+    rng = np.random.default_rng(seed)
+    layers = np.arange(1, spec.L - 1)
+    pick = rng.random(nl.n) < fraction
+    sp, sn, sc, tp, tn, tc = [0], [], [], [0], [], []
+
+    def cells(pad):
+        x, y, _ = spec.coords(np.asarray([pad]))
+        out = [(int(x[0]), int(y[0]))]
+        while len(out) < candidates:
+            cx = int(np.clip(out[0][0] + rng.integers(-2, 3), 0, spec.X - 1))
+            cy = int(np.clip(out[0][1] + rng.integers(-2, 3), 0, spec.Y - 1))
+            if (cx, cy) not in out:
+                out.append((cx, cy))
+        return out
+
+    def terms(pad):
+        nodes, costs = [], []
+        for c, (x, y) in enumerate(cells(pad)):
+            for z in layers:
+                nodes.append(int(z) * spec.XY + y * spec.X + x)
+                costs.append(np.float32(via_cost * discount * float(z) + candidate_penalty * c))
+        return nodes, costs
+
+    box = nl.box.copy()
+    margin = int(nl.meta.get("margin", 4))
+    for k in range(nl.n):
+        if pick[k]:
+            a, ac = terms(int(nl.src[k]))
+            b, bc = terms(int(nl.dst[k]))
+            if len(b) > MAX_TARGETS:
+                raise ValueError("too many targets per net")
+            sn += a; sc += ac; tn += b; tc += bc
+            xs, ys, _ = spec.coords(np.asarray(a + b, np.int64))
+            x0 = max(0, min(int(box[k, 0]), int(xs.min()) - margin))
+            y0 = max(0, min(int(box[k, 1]), int(ys.min()) - margin))
+            x1 = min(spec.X - 1, max(int(box[k, 0] + box[k, 2] - 1), int(xs.max()) + margin))
+            y1 = min(spec.Y - 1, max(int(box[k, 1] + box[k, 3] - 1), int(ys.max()) + margin))
+            box[k] = (x0, y0, x1 - x0 + 1, y1 - y0 + 1)
+        sp.append(len(sn))
+        tp.append(len(tn))
+    T = Terminals(np.asarray(sp, np.int32), np.asarray(sn, np.int32), np.asarray(sc, np.float32).reshape(-1),
+                  np.asarray(tp, np.int32), np.asarray(tn, np.int32), np.asarray(tc, np.float32).reshape(-1))
+    T.validate(nl.n, spec.N)
+    meta = dict(nl.meta, portals={"candidates": candidates, "fraction": fraction, "seed": seed})
+    return Netlist(spec, nl.src.copy(), nl.dst.copy(), box.astype(np.int32), f"{nl.name}_portals{candidates}", meta,
+                   T)

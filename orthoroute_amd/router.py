@@ -227,15 +227,28 @@ class NetTables:
         self.ws_off = torch.from_numpy(ws_off).to(dev)
         self.done = torch.full((max(1, n),), -1, dtype=torch.int32, device=dev)
         self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        # multi-terminal nets: seed / target tables (orx_nets.seed_ptr ...)
+        self.terms = []
+        tp = [0] * 6
+        if netlist.terms is not None:
+            T = netlist.terms
+            T.validate(n, netlist.spec.N)
+            zero = np.float32(0.0)   # + 0 turns -0.0 into +0.0 (keys order by fp32 bits)
+            arrs = [(T.seed_ptr, np.int32), (T.seed_node, np.int32), (T.seed_cost + zero, np.float32),
+                    (T.tgt_ptr, np.int32), (T.tgt_node, np.int32), (T.tgt_cost + zero, np.float32)]
+            self.terms = [torch.from_numpy(np.ascontiguousarray(a, t)).to(dev) if len(a) else
+                          torch.zeros(1, dtype=torch.int32 if t == np.int32 else torch.float32, device=dev)
+                          for a, t in arrs]
+            tp = [t.data_ptr() for t in self.terms]
         self.struct = _lib.OrxNets(
             n, self.box.data_ptr(), self.src.data_ptr(), self.dst.data_ptr(), self.path_arena.data_ptr(),
             self.path_off.data_ptr(), self.path_cap.data_ptr(), self.path_len.data_ptr(),
-            self.path_cost.data_ptr(), self.status.data_ptr())
+            self.path_cost.data_ptr(), self.status.data_ptr(), *tp)
 
     def nbytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in
                    (self.box, self.src, self.dst, self.path_arena, self.path_off, self.path_cap,
-                    self.path_len, self.path_cost, self.status, self.ws_off, self.done))
+                    self.path_len, self.path_cost, self.status, self.ws_off, self.done, *self.terms))
 
     def paths(self) -> list[np.ndarray]:
         arena = self.path_arena.cpu().numpy()

@@ -22,7 +22,7 @@ def build_driver() -> Path:
 
 
 class CpuBuildPathFinder:
-    def __init__(self, netlist, cfg, workdir: Path, mode: int = 0, max_groups: int = 0):
+    def __init__(self, netlist, cfg, workdir: Path, mode: int = 0, max_groups: int = 0, base=None):
         self.nl = netlist
         self.cfg = cfg
         self.spec = spec = netlist.spec
@@ -31,7 +31,7 @@ class CpuBuildPathFinder:
         self.mode = mode
         self.max_groups = max_groups
         N, n = spec.N, netlist.n
-        self.base = spec.base_costs()
+        self.base = spec.base_costs() if base is None else np.asarray(base, np.float32).copy()
         self.hist = np.zeros(4 * N, np.float32)
         self.occ = np.zeros(4 * N, np.int32)
         self.wcost = np.zeros(4 * N, np.float32)
@@ -96,6 +96,9 @@ class CpuBuildPathFinder:
             "astar_cp": self.astar[0], "astar_cv": self.astar[1], "astar_gmax": self.astar[2],
             "delta": c.delta, "block": c.block, "lds_nodes": c.lds_nodes, "flags": 1,
         }
+        T = self.nl.terms
+        if T is not None:
+            meta.update({"terms": 1, "n_seed": len(T.seed_node), "n_tgt": len(T.tgt_node)})
         with open(self.dir / "meta.txt", "w") as f:
             for k, v in meta.items():
                 f.write(f"{k} {float(v)!r}\n")
@@ -110,6 +113,10 @@ class CpuBuildPathFinder:
             "dep_ids": dep_ids.astype(np.int32) if len(dep_ids) else np.zeros(1, np.int32),
             "ws_off": self.ws_off, "done": self.done,
         }
+        if T is not None:
+            arrays.update({"seed_ptr": T.seed_ptr.astype(np.int32), "seed_node": T.seed_node.astype(np.int32),
+                           "seed_cost": T.seed_cost.astype(np.float32), "tgt_ptr": T.tgt_ptr.astype(np.int32),
+                           "tgt_node": T.tgt_node.astype(np.int32), "tgt_cost": T.tgt_cost.astype(np.float32)})
         for k, a in arrays.items():
             np.ascontiguousarray(a).tofile(self.dir / k)
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:halt_on_error=1")
@@ -131,7 +138,8 @@ class CpuBuildPathFinder:
         return {"routed": int(stats[5]), "failed": int(stats[6]), "edge_over": int(out[0]),
                 "edge_over_count": int(out[1]), "node_over": int(out[2]), "node_over_count": int(out[3]),
                 "via_hops": int(out[4]), "barrels": int(out[5]), "settled": int(stats[0]), "relaxed": int(stats[1]),
-                "path_hops": int(stats[7]), "rounds": int(stats[3]), "phases": int(stats[4])}
+                "path_hops": int(stats[7]), "rounds": int(stats[3]), "phases": int(stats[4]),
+                "status": self.status.copy()}
 
     def paths(self):
         return [self.arena[o:o + l].copy() for o, l in zip(self.path_off, self.path_len)]

@@ -223,6 +223,13 @@ template <typename T, typename U>
 T atomicAdd(T* p, U v) { return std::atomic_ref<T>(*p).fetch_add((T)v); }
 template <typename T, typename U>
 T atomicMin(T* p, U v) { return __hip_atomic_fetch_min(p, (T)v, 0, 0); }
+template <typename T, typename U>
+T atomicMax(T* p, U v) {
+    std::atomic_ref<T> a(*p);
+    T cur = a.load();
+    while ((T)v > cur && !a.compare_exchange_weak(cur, (T)v)) {}
+    return cur;
+}
 
 // runtime API
 typedef int hipError_t;

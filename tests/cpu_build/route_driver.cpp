@@ -104,6 +104,19 @@ int main(int argc, char** argv) {
                  (float)Mf("astar_cp"), (float)Mf("astar_cv"), (float)Mf("astar_gmax")};
     orx_nets nets{n, box.data(), src.data(), dst.data(), arena.data(), path_off.data(), path_cap.data(),
                   path_len.data(), path_cost.data(), status.data()};
+    // multi-terminal tables, each of exactly its size
+    std::vector<int32_t> seed_ptr, seed_node, tgt_ptr, tgt_node;
+    std::vector<float> seed_cost, tgt_cost;
+    if (g_meta.count("terms") && M("terms")) {
+        seed_ptr = load<int32_t>("seed_ptr", (size_t)n + 1);
+        tgt_ptr = load<int32_t>("tgt_ptr", (size_t)n + 1);
+        seed_node = load<int32_t>("seed_node", (size_t)M("n_seed"));
+        seed_cost = load<float>("seed_cost", (size_t)M("n_seed"));
+        tgt_node = load<int32_t>("tgt_node", (size_t)M("n_tgt"));
+        tgt_cost = load<float>("tgt_cost", (size_t)M("n_tgt"));
+        nets.seed_ptr = seed_ptr.data(); nets.seed_node = seed_node.data(); nets.seed_cost = seed_cost.data();
+        nets.tgt_ptr = tgt_ptr.data(); nets.tgt_node = tgt_node.data(); nets.tgt_cost = tgt_cost.data();
+    }
     orx_workspace ws{keys.data(), queues.data(), bitmaps.data(), wsn};
     orx_route_cfg cfg{(float)Mf("delta"), (int32_t)M("block"), (int32_t)M("lds_nodes"), (int32_t)M("flags"),
                       stats.data()};

@@ -120,3 +120,68 @@ def test_grid_search_on_host_threads(driver, tmp_path):
     od, op, _, _, _ = oracle_sssp_box(spec, w, box, src)
     assert np.array_equal(dist.view(np.uint32), od.view(np.uint32))
     np.testing.assert_array_equal(parent, op)
+
+
+@pytest.mark.parametrize("mode,lds_nodes", [(2, 0), (1, 12288)])
+def test_portal_nets_on_host_threads(driver, tmp_path, monkeypatch, mode, lds_nodes):
+    """Multi-terminal nets (seeds and targets on every inner layer of two
+    escape cells per pad) mixed with two-pin nets: paths, fp32 totals,
+    occupancy, history and search counts equal the oracle's."""
+    from orthoroute_amd.netgen import with_portals
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "2")
+    nl = with_portals(uneven(24, 20, 4, 10, seed=4), candidates=2, fraction=0.7, seed=1)
+    assert 0 < int((nl.terms.counts()[0] > 0).sum()) < nl.n
+    cfg = RouterConfig(iterations=2, lds_nodes=lds_nodes)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=mode)
+    o = OraclePathFinder(nl, cfg)
+    for _ in range(2):
+        d = c.run_iteration(timeout=300)
+        od = o.run_iteration()
+        for k in ("routed", "failed", "edge_over", "node_over", "via_hops", "barrels", "settled", "relaxed",
+                  "path_hops"):
+            assert d[k] == od[k], k
+        for a, b in zip(c.paths(), o.paths()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(c.path_cost.view(np.uint32), o.path_cost.view(np.uint32))
+        assert np.array_equal(c.occ, o.occ)
+        assert np.array_equal(c.node_use, o.node_use)
+        assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
+
+
+def test_nets_without_a_path_commit_nothing(driver, tmp_path):
+    """A two-pin net whose sink is walled off, and a multi-terminal net whose
+    best target is one of its seeds: both end with ORX_NO_PATH, an empty
+    path and nothing committed, as in the oracle; the other nets route."""
+    from orthoroute_amd.netgen import Netlist, Terminals
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    nl2 = uneven(24, 20, 3, 6, seed=9)
+    spec = nl2.spec
+    base = spec.base_costs()
+    nb = spec.neighbours()
+    sink = int(nl2.dst[0])
+    for s in range(4):                      # every arc into net 0's sink
+        u = int(nb[sink, s])
+        if u >= 0:
+            base[4 * u + (3 - s)] = np.inf
+    n = nl2.n
+    a = int(spec.node(*[int(v[0]) for v in spec.coords(np.asarray([nl2.src[1]]))][:2], 1))
+    c2 = int(spec.node(*[int(v[0]) for v in spec.coords(np.asarray([nl2.dst[1]]))][:2], 1))
+    seed_ptr = np.zeros(n + 1, np.int32)
+    seed_ptr[2:] = 2                        # net 1: seeds {src pad, a}, targets {a, dst cell on layer 1}
+    T = Terminals(seed_ptr, np.asarray([int(nl2.src[1]), a], np.int32), np.zeros(2, np.float32),
+                  seed_ptr.copy(), np.asarray([a, c2], np.int32), np.zeros(2, np.float32))
+    nl = Netlist(spec, nl2.src, nl2.dst, nl2.box, "nopath", dict(nl2.meta), T)
+    cfg = RouterConfig(iterations=1)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=2, base=base)
+    o = OraclePathFinder(nl, cfg, base=base)
+    d = c.run_iteration(timeout=300)
+    od = o.run_iteration()
+    assert d["failed"] == od["failed"] == 2
+    assert d["status"][0] == 1 and d["status"][1] == 1      # ORX_NO_PATH
+    assert c.path_len[0] == 0 and c.path_len[1] == 0
+    for a_, b_ in zip(c.paths(), o.paths()):
+        assert np.array_equal(a_, b_)
+    assert np.array_equal(c.occ, o.occ)
+    assert np.array_equal(c.node_use, o.node_use)
+    assert np.array_equal(c.via_col, o.via_col)

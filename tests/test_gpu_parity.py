@@ -154,3 +154,66 @@ def test_sssp_grid_random_weights(device, delta):
         od, op, _, ns, nr = oracle_sssp_box(spec, w, box, int(src))
         assert np.array_equal(dist.cpu().numpy().view(np.uint32), od.view(np.uint32))
         np.testing.assert_array_equal(parent.cpu().numpy(), op)
+
+
+@pytest.mark.parametrize("lds_nodes,block", [(0, 1024), (16384, 1024), (12288, 256)])
+def test_portal_nets_bit_exact(device, lds_nodes, block):
+    """Multi-terminal nets (find_path_multisource_multisink semantics): every
+    pad a seed or target on each inner layer of two escape cells, mixed with
+    two-pin nets, three iterations."""
+    from orthoroute_amd.netgen import with_portals
+    nl = with_portals(uneven(96, 80, 8, 300, seed=13), candidates=2, fraction=0.8, seed=3)
+    cfg = RouterConfig(iterations=3, delta=16.0, block=block, lds_nodes=lds_nodes)
+    _run_both(nl, cfg, 3, "batched", device)
+
+
+def test_portal_nets_plain_mode_bit_exact(device):
+    from orthoroute_amd.netgen import with_portals
+    nl = with_portals(uneven(64, 64, 6, 150, seed=5), candidates=3, fraction=1.0, seed=4)
+    cfg = RouterConfig(iterations=2)
+    _run_both(nl, cfg, 2, "plain", device)
+
+
+def test_portal_largest_rois_bit_exact(device):
+    """Portal form of the 16 largest nets of the 800x800x20 board (54
+    targets per net, search state in the global workspace)."""
+    from orthoroute_amd.netgen import uneven_800, with_portals
+    nl = uneven_800()
+    big = np.argsort(-nl.roi_nodes(), kind="stable")[:16]
+    sub = with_portals(nl.select(np.sort(big), "largest16"), candidates=3, seed=5)
+    cfg = RouterConfig(iterations=1, delta=16.0, block=1024, lds_nodes=16384)
+    _run_both(sub, cfg, 1, "batched", device)
+
+
+def test_nets_without_a_path_commit_nothing(device):
+    """A two-pin net whose sink is walled off and a multi-terminal net whose
+    best target is one of its seeds end with ORX_NO_PATH and commit nothing,
+    as in the oracle."""
+    from orthoroute_amd.netgen import Netlist, Terminals
+    nl2 = uneven(24, 20, 3, 6, seed=9)
+    spec = nl2.spec
+    base = spec.base_costs()
+    nb = spec.neighbours()
+    sink = int(nl2.dst[0])
+    for s in range(4):
+        u = int(nb[sink, s])
+        if u >= 0:
+            base[4 * u + (3 - s)] = np.inf
+    n = nl2.n
+    sx, sy, _ = spec.coords(np.asarray([nl2.src[1]]))
+    tx, ty, _ = spec.coords(np.asarray([nl2.dst[1]]))
+    a, c2 = int(spec.node(int(sx[0]), int(sy[0]), 1)), int(spec.node(int(tx[0]), int(ty[0]), 1))
+    seed_ptr = np.zeros(n + 1, np.int32)
+    seed_ptr[2:] = 2
+    T = Terminals(seed_ptr, np.asarray([int(nl2.src[1]), a], np.int32), np.zeros(2, np.float32),
+                  seed_ptr.copy(), np.asarray([a, c2], np.int32), np.zeros(2, np.float32))
+    nl = Netlist(spec, nl2.src, nl2.dst, nl2.box, "nopath", dict(nl2.meta), T)
+    cfg = RouterConfig(iterations=1)
+    dev = DevicePathFinder(nl, cfg, device=device, base=base)
+    orc = OraclePathFinder(nl, cfg, base=base)
+    sd = dev.run_iteration("batched")
+    so = orc.run_iteration()
+    _assert_same_iteration(sd, so)
+    _assert_same_state(dev, orc)
+    st = dev.status()
+    assert st[0] == 1 and st[1] == 1 and sd.failed == 2
