@@ -1773,7 +1773,8 @@ int orx_lattice_end_iteration(const orx_lattice* lat, const orx_params* p, int64
     if (!p || !d_out) return orx_host::fail(ORX_E_ARG, "params / out is NULL");
     const int64_t N = (int64_t)lat->X * lat->Y * lat->L;
     hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, (unsigned long long*)d_out, 8);
-    const int grid = (int)std::min<int64_t>((N + 255) / 256, 8192);
+    // grid-stride over 32 workgroups per CU at most (8192 on MI355X)
+    const int grid = (int)std::min<int64_t>((N + 255) / 256, 32ll * device_cus());
     hipLaunchKernelGGL(end_iteration_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, make_lat(lat),
                        make_params(p), N, (unsigned long long*)d_out);
     ORX_HIP_CHECK(hipGetLastError());

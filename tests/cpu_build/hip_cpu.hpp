@@ -134,10 +134,18 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
     unsigned conc = 1;
     if (const char* e = getenv("HIPCPU_CONCURRENT_WG")) conc = (unsigned)atoi(e);
     if (conc < 1) conc = 1;
-    for (unsigned b0 = 0; b0 < grid.x; b0 += conc) {
-        const unsigned nb = std::min(conc, grid.x - b0);
-        std::vector<std::unique_ptr<Block>> blks;
+    if (grid.x == 0) return;
+    // One pool of nb * blockDim threads runs the launch in steps of nb
+    // workgroups; thread (0, 0) builds each step's workgroup state between
+    // two pool-wide barriers, so every step starts after the previous one
+    // has ended, as with a fresh set of threads per step.
+    const unsigned nb = std::min(conc, grid.x);
+    std::vector<std::unique_ptr<Block>> blks(nb);
+    std::barrier<> pool((ptrdiff_t)nb * block.x);
+    auto build = [&](unsigned b0) {
         for (unsigned j = 0; j < nb; ++j) {
+            blks[j].reset();
+            if (b0 + j >= grid.x) continue;
             auto blk = std::make_unique<Block>();
             blk->idx = {b0 + j, 0, 0};
             blk->dyn.reset(new unsigned char[lds ? lds : 1]);
@@ -149,19 +157,26 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
                 blk->wave_bar.push_back(std::make_unique<std::barrier<>>((ptrdiff_t)(n > 64 ? 64 : n)));
             }
             blk->exch.assign(64 * nw, 0);
-            blks.push_back(std::move(blk));
+            blks[j] = std::move(blk);
         }
-        std::vector<std::thread> th;
-        th.reserve((size_t)nb * block.x);
-        for (unsigned j = 0; j < nb; ++j)
-            for (unsigned t = 0; t < block.x; ++t)
-                th.emplace_back([&, j, t]() {
-                    t_thread = {t, 0, 0};
-                    t_blk = blks[j].get();
-                    kernel(args...);
-                });
-        for (auto& x : th) x.join();
-    }
+    };
+    std::vector<std::thread> th;
+    th.reserve((size_t)nb * block.x);
+    for (unsigned j = 0; j < nb; ++j)
+        for (unsigned t = 0; t < block.x; ++t)
+            th.emplace_back([&, j, t]() {
+                for (unsigned b0 = 0; b0 < grid.x; b0 += nb) {
+                    if (j == 0 && t == 0) build(b0);
+                    pool.arrive_and_wait();
+                    if (blks[j]) {
+                        t_thread = {t, 0, 0};
+                        t_blk = blks[j].get();
+                        kernel(args...);
+                    }
+                    pool.arrive_and_wait();
+                }
+            });
+    for (auto& x : th) x.join();
 }
 }  // namespace hipcpu
 
@@ -223,6 +238,12 @@ template <typename T, typename U>
 T atomicAdd(T* p, U v) { return std::atomic_ref<T>(*p).fetch_add((T)v); }
 template <typename T, typename U>
 T atomicMin(T* p, U v) { return __hip_atomic_fetch_min(p, (T)v, 0, 0); }
+template <typename T, typename U, typename V>
+T atomicCAS(T* p, U expected, V desired) {
+    T cur = (T)expected;
+    std::atomic_ref<T>(*p).compare_exchange_strong(cur, (T)desired);
+    return cur;
+}
 template <typename T, typename U>
 T atomicMax(T* p, U v) {
     std::atomic_ref<T> a(*p);
