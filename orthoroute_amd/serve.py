@@ -61,6 +61,9 @@ class RouteJob:
     footprint: int = 0
     nets_routed: int = 0
     gpu_ms: float = 0.0
+    # called as on_finish(job, runner) when the job retires, before its
+    # router is dropped (for example boardio.ors_writer: paths -> .ORS)
+    on_finish: Callable | None = None
 
     @property
     def latency(self) -> float:
@@ -160,6 +163,8 @@ class RoutingServer:
     def _retire(self, job: RouteJob, runner, slot: int) -> None:
         job.digest = runner.digest()
         job.finished_at = self.now()
+        if job.on_finish is not None:
+            job.on_finish(job, runner)
         del self.active[job.job_id]
         self.in_use -= job.footprint
         self.free_slots.append(slot)

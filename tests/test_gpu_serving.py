@@ -74,3 +74,26 @@ def test_graph_captured_iteration_equals_direct(device):
         assert a.digest() == b.digest()
     pa, pb = a.paths(), b.paths()
     assert all(np.array_equal(x, y) for x, y in zip(pa, pb))
+
+
+@pytest.mark.parametrize("portals", [0, 1])
+def test_board_file_routes_like_the_oracle(device, tmp_path, portals):
+    """A board read from an ORP file (two-pin pads, or portal terminals on
+    every inner layer) routed on the device: its ORS geometry equals the
+    oracle's for the same board, four iterations."""
+    from oracle.oracle import OraclePathFinder
+    from orthoroute_amd.boardio import load_orp, netlist_to_orp, orp_to_board, save_orp, solution_to_ors
+    save_orp(netlist_to_orp(uneven(96, 80, 8, 300, seed=31), name="file_board"), tmp_path / "b.orp")
+    board = orp_to_board(load_orp(tmp_path / "b.orp"), portals=portals)
+    cfg = RouterConfig(iterations=4, delta=16.0, block=1024, lds_nodes=16384)
+    dev = DevicePathFinder(board.netlist, cfg, device=device)
+    orc = OraclePathFinder(board.netlist, cfg)
+    for _ in range(4):
+        print(f"[board] portals={portals} iteration {dev.iteration + 1}", flush=True)
+        sd = dev.run_iteration("batched")
+        so = orc.run_iteration()
+        assert sd.overuse == so["edge_over"] + so["node_over"] and sd.routed == so["routed"]
+    got = solution_to_ors(board, dev.paths(), dev.history, 0.0)
+    want = solution_to_ors(board, orc.paths(), [], 0.0)
+    assert got["geometry"] == want["geometry"]
+    assert got["statistics"]["total_wirelength_mm"] == want["statistics"]["total_wirelength_mm"]
