@@ -264,6 +264,33 @@ def portal_bench(nl, cfg, candidates: int, steps: int, cpu_stride: int, with_cpu
     return out
 
 
+def convergence_bench(nl, cfg, max_iterations: int) -> dict:
+    """Iterations and wall time from scratch to zero overuse, rerouting every
+    net (the plain schedule) and in hotset mode (offenders only)."""
+    import dataclasses
+
+    import torch
+    from orthoroute_amd.router import DevicePathFinder
+    out = {}
+    for name, hot in (("reroute_all", False), ("hotset", True)):
+        c = dataclasses.replace(cfg, iterations=max_iterations, collect_stats=False, hotset=hot)
+        dev = DevicePathFinder(nl, c)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        routed = 0
+        for _ in range(max_iterations):
+            st = dev.run_iteration("batched")
+            routed += st.routed
+            if st.overuse == 0:
+                break
+        torch.cuda.synchronize()
+        out[name] = {"iterations": dev.iteration, "seconds": time.perf_counter() - t0, "nets_rerouted": routed,
+                     "overuse_last": st.overuse, "converged": st.overuse == 0}
+        del dev
+        torch.cuda.empty_cache()
+    return out
+
+
 def _traffic():
     """HBM bytes per route_wave_kernel dispatch from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
@@ -291,6 +318,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-iterations", type=int, default=50)
     ap.add_argument("--serve-seed", type=int, default=4)
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--converge-iterations", type=int, default=30)
     ap.add_argument("--no-portal", action="store_true")
     ap.add_argument("--portal-candidates", type=int, default=1)
     ap.add_argument("--portal-steps", type=int, default=2)
@@ -399,6 +428,12 @@ def main() -> int:
                                 time_limit=300.0)
         torch.cuda.empty_cache()
 
+    converge = None
+    if not args.no_converge:
+        _log(f"convergence: up to {args.converge_iterations} iterations, reroute-all and hotset")
+        converge = convergence_bench(nl, cfg, args.converge_iterations)
+        _log("convergence: " + ", ".join(f"{k} {v['iterations']} it {v['seconds']:.2f} s" for k, v in converge.items()))
+
     portal = None
     if not args.no_portal and nl.spec.L >= 3:
         _log(f"portal form: {args.portal_candidates} escape cell(s) per pad, {args.portal_steps} timed iterations")
@@ -450,6 +485,7 @@ def main() -> int:
         "sssp_microbench": sssp,
         "serving": serving,
         "portal_nets": portal,
+        "convergence": converge,
     }
     print(json.dumps(out))
     return 0

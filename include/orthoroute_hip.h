@@ -233,19 +233,25 @@ int64_t orx_plan_iteration(const int32_t* box, const int32_t* order, int32_t n, 
                            int32_t Y, const int64_t* roi_nodes, int32_t* wave, int32_t* sched,
                            int32_t* dep_ptr, int32_t* dep_ids, int64_t dep_cap);
 
-/* Host planner, waves only: the same wave[n] as orx_plan_iteration and the
+/* Host planner, waves only: the same waves as orx_plan_iteration and the
  * schedule sched[n] (by wave, larger ROI first, then routing order),
  * without dependency lists, on a tiled grid (8x8-cell tiles) so large boxes
- * cost their tile count plus their border cells.  Returns the number of
- * waves, or < 0 on bad input.  No device work. */
-int64_t orx_plan_waves(const int32_t* box, const int32_t* order, int32_t n, int32_t X, int32_t Y,
+ * cost their tile count plus their border cells.  order[n] lists distinct
+ * net ids below n_nets (all nets, or a hotset); box, roi_nodes and wave are
+ * indexed by net id (n_nets entries; only the ordered nets' wave entries are
+ * written).  Returns the number of waves, or < 0 on bad input.  No device
+ * work. */
+int64_t orx_plan_waves(const int32_t* box, int32_t n_nets, const int32_t* order, int32_t n, int32_t X, int32_t Y,
                        const int64_t* roi_nodes, int32_t* wave, int32_t* sched);
 
 /* Per-net overuse of its committed path, summed over the forward arcs
  * (d_over[n]) and its hop count (d_hops[n]): the congestion input of
- * _order_nets_by_difficulty (UP:7840-7901). */
-int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_cap,
-                    int64_t* d_over, int32_t* d_hops, orx_stream_t stream);
+ * _order_nets_by_difficulty (UP:7840-7901).  d_node_over[n] (NULL to skip)
+ * receives the overuse max(0, use - node_cap) summed over the path's nodes:
+ * with d_over it marks the nets that hold an over-capacity resource, the
+ * offenders a hotset reroutes (_build_hotset UP:9143-9171). */
+int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_cap, int32_t node_cap,
+                    int64_t* d_over, int64_t* d_node_over, int32_t* d_hops, orx_stream_t stream);
 
 /* Commit (sign=+1) or rip up (sign=-1) K device paths into the occupancy
  * arrays and re-price the touched arcs (EdgeAccountant.commit_path /

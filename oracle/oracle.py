@@ -172,9 +172,20 @@ class OraclePathFinder:
         else:
             over = np.zeros(self.netlist.n, np.int64)
             hops = np.zeros(self.netlist.n, np.int32)
-        return net_order(iteration, self.distance, over, hops)
+        order = net_order(iteration, self.distance, over, hops)
+        h = self.history
+        stalled = len(h) >= 2 and (h[-1]["edge_over"] + h[-1]["node_over"] >= h[-2]["edge_over"] + h[-2]["node_over"])
+        if getattr(self.cfg, "hotset", False) and iteration > 1 and not stalled:
+            # offenders only (_build_hotset, unified_pathfinder.py:9143-9171):
+            # a committed path on an over-capacity edge or node, or no path;
+            # every net after an iteration that did not lower the overuse
+            node_over = np.array([int(np.maximum(self.node_use[p] - self.cfg.node_cap, 0).sum())
+                                  for p in self.paths()], np.int64)
+            keep = [k for k in order if over[k] > 0 or node_over[k] > 0 or hops[k] == 0]
+            order = np.asarray(keep, np.int32)
+        return order
 
-    def run_iteration(self, order: np.ndarray | None = None, nets_limit: int = -1) -> dict:
+    def run_iteration(self, order: np.ndarray | None = None) -> dict:
         it = self.iteration + 1
         if order is None:
             order = self.order_for(it)
@@ -189,7 +200,7 @@ class OraclePathFinder:
             s.X, s.Y, s.L, s.hmask, _p(self.base), self.netlist.n, _p(self.src), _p(self.dst), _p(self.box),
             1, _p(order), _p(pres), C.byref(P), _p(self.hist), _p(self.occ), _p(self.node_hist),
             _p(self.node_use), _p(self.via_col), _p(self.arena), _p(self.path_off), _p(self.path_cap),
-            _p(self.path_len), _p(self.path_cost), _p(stats), int(nets_limit),
+            _p(self.path_len), _p(self.path_cost), _p(stats), len(order),
             None if self.terms is None else C.byref(self.terms))
         if rc < 0:
             raise RuntimeError(f"oracle pathfinder failed: {rc}")

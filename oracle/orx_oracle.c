@@ -687,8 +687,8 @@ void orx_oracle_net_overuse(int32_t X, int32_t Y, int32_t L, uint32_t hmask, con
 /* For each iteration it: for each net in order[it]: rip up its old path,    */
 /* route it in its ROI with live costs, commit.  Then overuse stats and the  */
 /* history update.  pres[it] is the present factor of iteration it.          */
-/* nets_limit >= 0 stops after that many nets of the first iteration (a      */
-/* bounded CPU-baseline sample).                                              */
+/* n_order: entries of order[] per iteration, the nets routed in it (a     */
+/* hotset routes a subset); < 0 means n_nets.                                */
 /* stats per iteration (16 int64): 0 routed, 1 failed, 2 edge_over,          */
 /* 3 edge_over_count, 4 node_over, 5 node_over_count, 6 via_hops, 7 barrels, */
 /* 8 settled nodes, 9 relaxed arcs, 10 total path hops.                      */
@@ -751,23 +751,21 @@ int orx_oracle_pathfinder(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const
                           float* hist, int32_t* occ, float* node_hist, int32_t* node_use,
                           int32_t* via_col, int32_t* path_arena, const int64_t* path_off,
                           const int64_t* path_cap, int32_t* path_len, float* dist_out,
-                          int64_t* stats, int64_t nets_limit, const orx_terms* terms) {
+                          int64_t* stats, int64_t n_order, const orx_terms* terms) {
     lat_t g = { X, Y, L, hmask };
     int64_t N = (int64_t)X * Y * L;
     sssp_ws W; memset(&W, 0, sizeof(W));
     orx_params P = *P0;
     int rc = 0;
-    int64_t budget = nets_limit;
+    const int64_t per = (n_order < 0 || n_order > n_nets) ? n_nets : n_order;
     for (int32_t it = 0; it < iters; ++it) {
         int64_t* st = stats + (int64_t)it * 16;
         memset(st, 0, 16 * sizeof(int64_t));
         P.pres_fac = pres[it];
         P.node_pres = P0->node_pres * pres[it];
         lat_state S = { base, hist, occ, node_hist, node_use, NULL };
-        for (int32_t i = 0; i < n_nets; ++i) {
-            if (budget == 0) goto done;
-            if (budget > 0) --budget;
-            int32_t k = order[(int64_t)it * n_nets + i];
+        for (int64_t i = 0; i < per; ++i) {
+            int32_t k = order[(int64_t)it * per + i];
             int r = route_one(&g, &S, &P, k, src, dst, box, terms, occ, node_use, via_col, path_arena, path_off,
                               path_cap, path_len, dist_out, &W, st);
             if (r < 0) { rc = r; goto done; }

@@ -104,8 +104,8 @@ SIGNATURES = [
                                 P, C.c_int32, P, P, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg),
                                 C.c_int32, P]),
     ("orx_plan_iteration", C.c_int64, [P, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int64]),
-    ("orx_plan_waves", C.c_int64, [P, P, C.c_int32, C.c_int32, C.c_int32, P, P, P]),
-    ("orx_net_overuse", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxNets), C.c_int32, P, P, P]),
+    ("orx_plan_waves", C.c_int64, [P, C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P]),
+    ("orx_net_overuse", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxNets), C.c_int32, C.c_int32, P, P, P, P]),
     ("orx_apply_paths", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), P, P, P, C.c_int32,
                                   C.c_int32, P]),
     ("orx_sssp_box", C.c_int, [C.POINTER(OrxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64,

@@ -1205,20 +1205,28 @@ __global__ __launch_bounds__(BLOCK) void reprice_paths_kernel(Lat g, Params P, c
 // Per-net overuse over the forward arcs of its committed path: the
 // numerator of _order_nets_by_difficulty's congestion term
 // (unified_pathfinder.py:7858-7863; both arcs of a hop hold equal occupancy).
-__global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, int32_t edge_cap, int64_t* out_over,
+__global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, int32_t edge_cap, int32_t node_cap,
+                                                          int64_t* out_over, int64_t* out_node_over,
                                                           int32_t* out_hops) {
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nets.n) return;
     const int32_t len = nets.path_len[k];
     const int32_t* path = nets.path_arena + nets.path_off[k];
-    int64_t over = 0;
-    for (int32_t i = 0; i + 1 < len; ++i) {
+    int64_t over = 0, node_over = 0;
+    for (int32_t i = 0; i < len; ++i) {
         const int64_t u = path[i];
-        const int s = lat_slot(g, u, path[i + 1]);
-        const int32_t o = g.occ[4 * u + s] - edge_cap;
-        over += o > 0 ? o : 0;
+        if (out_node_over) {
+            const int32_t o = g.node_use[u] - node_cap;
+            node_over += o > 0 ? o : 0;
+        }
+        if (i + 1 < len) {
+            const int s = lat_slot(g, u, path[i + 1]);
+            const int32_t o = g.occ[4 * u + s] - edge_cap;
+            over += o > 0 ? o : 0;
+        }
     }
     out_over[k] = over;
+    if (out_node_over) out_node_over[k] = node_over;
     out_hops[k] = len > 0 ? len - 1 : 0;
 }
 
@@ -1877,15 +1885,15 @@ int orx_route_dag(const orx_lattice* lat, const orx_params* p, const orx_nets* n
     return ORX_OK;
 }
 
-int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_cap, int64_t* d_over,
-                    int32_t* d_hops, orx_stream_t stream) {
+int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_cap, int32_t node_cap,
+                    int64_t* d_over, int64_t* d_node_over, int32_t* d_hops, orx_stream_t stream) {
     int rc = check_lattice(lat);
     if (rc) return rc;
     if (!nets || !d_over || !d_hops) return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (nets->n <= 0) return ORX_OK;
     const NetsDev nd = make_nets(nets);
     hipLaunchKernelGGL(net_overuse_kernel, dim3((nets->n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       make_lat(lat), nd, edge_cap, d_over, d_hops);
+                       make_lat(lat), nd, edge_cap, node_cap, d_over, d_node_over, d_hops);
     ORX_HIP_CHECK(hipGetLastError());
     return ORX_OK;
 }

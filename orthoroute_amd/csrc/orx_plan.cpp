@@ -86,9 +86,11 @@ extern "C" int64_t orx_plan_iteration(const int32_t* box, const int32_t* order, 
 // the tiles its border crosses.  A cell's wave is max(cell, tfull of its
 // tile).  Every box's wave exceeds every value inside it, so an update only
 // ever raises values.
-extern "C" int64_t orx_plan_waves(const int32_t* box, const int32_t* order, int32_t n, int32_t X, int32_t Y,
-                                  const int64_t* roi_nodes, int32_t* wave, int32_t* sched) {
-    if (n < 0 || X <= 0 || Y <= 0) return -1;
+extern "C" int64_t orx_plan_waves(const int32_t* box, int32_t n_nets, const int32_t* order, int32_t n, int32_t X,
+                                  int32_t Y, const int64_t* roi_nodes, int32_t* wave, int32_t* sched) {
+    if (n < 0 || n > n_nets || X <= 0 || Y <= 0) return -1;
+    for (int32_t i = 0; i < n; ++i)
+        if (order[i] < 0 || order[i] >= n_nets) return -3;
     constexpr int32_t T = 8;
     const int32_t TX = (X + T - 1) / T, TY = (Y + T - 1) / T;
     std::vector<int32_t> cell((size_t)X * Y, -1), tfull((size_t)TX * TY, -1), tmax((size_t)TX * TY, -1);

@@ -59,6 +59,7 @@ class RouterConfig:
     collect_stats: bool = True
     graph: bool = False               # capture each iteration's wave launches in one hipGraph
     ws_cap_nodes: int = 1 << 24       # search workspace bound: larger waves launch in parts
+    hotset: bool = False              # reroute only offenders (hotset_filter) unless the overuse stalled
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -291,6 +292,19 @@ def _wave_offsets(roi_nodes: np.ndarray, nets: np.ndarray) -> tuple[np.ndarray, 
     return off, int(r.sum())
 
 
+def hotset_filter(order: np.ndarray, edge_over: np.ndarray, node_over: np.ndarray,
+                  hops: np.ndarray) -> np.ndarray:
+    """The nets of ``order`` (kept in that order) that the hotset reroutes:
+    those whose committed path holds an over-capacity edge or node, and
+    those without a path (_build_hotset, unified_pathfinder.py:9143-9171:
+    offenders on present overuse plus path-node conflicts, and unrouted
+    nets (0 hops); its caps, freezing and random exploration are not part
+    of it).  The router applies it from iteration 2 on, except after an
+    iteration that did not lower the overuse, when every net is rerouted."""
+    hot = (np.asarray(edge_over) > 0) | (np.asarray(node_over) > 0) | (np.asarray(hops) == 0)
+    return order[hot[order]]
+
+
 def split_waves(waves: list[np.ndarray], roi_nodes: np.ndarray, cap_nodes: int) -> list[np.ndarray]:
     """Split every wave whose 64-aligned ROI total exceeds ``cap_nodes`` into
     consecutive parts that fit (nets of one wave are independent, so any
@@ -388,6 +402,7 @@ class DevicePathFinder:
             self._route_stats = torch.zeros(8, dtype=torch.int64, device=self.device)
             self._iter_out = torch.zeros(8, dtype=torch.int64, device=self.device)
             self._over = torch.zeros(max(1, n), dtype=torch.int64, device=self.device)
+            self._node_over = torch.zeros(max(1, n), dtype=torch.int64, device=self.device)
             self._hops = torch.zeros(max(1, n), dtype=torch.int32, device=self.device)
             self._plan_nets = torch.empty(max(1, n), dtype=torch.int32, device=self.device)
             self._plan_off = torch.empty(max(1, n), dtype=torch.int64, device=self.device)
@@ -396,6 +411,7 @@ class DevicePathFinder:
         self._iter_out_h = torch.zeros(8, dtype=torch.int64, pin_memory=True)
         self._route_stats_h = torch.zeros(8, dtype=torch.int64, pin_memory=True)
         self._over_h = torch.zeros(max(1, n), dtype=torch.int64, pin_memory=True)
+        self._node_over_h = torch.zeros(max(1, n), dtype=torch.int64, pin_memory=True)
         self._hops_h = torch.zeros(max(1, n), dtype=torch.int32, pin_memory=True)
         self._plan_nets_h = torch.empty(max(1, n), dtype=torch.int32, pin_memory=True)
         self._plan_off_h = torch.empty(max(1, n), dtype=torch.int64, pin_memory=True)
@@ -456,8 +472,10 @@ class DevicePathFinder:
         n = self.netlist.n
         with torch.cuda.stream(self.stream):
             _lib.call("orx_net_overuse", C.byref(self.lat.struct), C.byref(self.nets.struct),
-                      self.cfg.edge_cap, self._over.data_ptr(), self._hops.data_ptr(), self._s())
+                      self.cfg.edge_cap, self.cfg.node_cap, self._over.data_ptr(), self._node_over.data_ptr(),
+                      self._hops.data_ptr(), self._s())
             self._over_h[:n].copy_(self._over[:n], non_blocking=True)
+            self._node_over_h[:n].copy_(self._node_over[:n], non_blocking=True)
             self._hops_h[:n].copy_(self._hops[:n], non_blocking=True)
 
     def order_for(self, iteration: int) -> np.ndarray:
@@ -474,7 +492,12 @@ class DevicePathFinder:
         else:
             over = np.zeros(n, np.int64)
             hops = np.zeros(n, np.int32)
-        return net_order(iteration, self.distance, over, hops)
+        order = net_order(iteration, self.distance, over, hops)
+        h = self.history
+        stalled = len(h) >= 2 and h[-1].overuse >= h[-2].overuse
+        if self.cfg.hotset and iteration > 1 and not stalled:
+            order = hotset_filter(order, over, self._node_over_h[:n].numpy(), hops)
+        return order
 
     def plan(self, order: np.ndarray, mode: str) -> list[np.ndarray]:
         """Launch groups: one per net (plain) or one per disjoint-ROI wave

@@ -58,9 +58,9 @@ def net_order(iteration: int, distance: np.ndarray, over_sum: np.ndarray, hops: 
 
 
 def color_waves(box: np.ndarray, order: np.ndarray, X: int, Y: int) -> np.ndarray:
-    """Wave index (0-based) of every net, given the routing order."""
+    """Wave index (0-based) of every net in ``order`` (-1 for the others)."""
     stamp = np.full((Y, X), -1, dtype=np.int32)
-    wave = np.empty(len(box), np.int32)
+    wave = np.full(len(box), -1, np.int32)
     for k in order:
         x0, y0, w, h = (int(v) for v in box[k])
         win = stamp[y0:y0 + h, x0:x0 + w]
@@ -73,7 +73,7 @@ def color_waves(box: np.ndarray, order: np.ndarray, X: int, Y: int) -> np.ndarra
 def group_waves(wave: np.ndarray, order: np.ndarray, roi_nodes: np.ndarray):
     """Lists of net ids per wave; inside a wave the largest ROI comes first
     (long jobs start first on the device)."""
-    n_waves = int(wave.max()) + 1 if len(wave) else 0
+    n_waves = int(wave[order].max()) + 1 if len(order) else 0
     buckets = [[] for _ in range(n_waves)]
     for k in order:
         buckets[wave[k]].append(int(k))
@@ -107,6 +107,8 @@ def plan_iteration(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes
     lib = _lib.load()
     n = len(order)
     box = np.ascontiguousarray(box.reshape(-1), np.int32)
+    if len(box) != 4 * n:
+        raise ValueError("plan_iteration needs an order of every net (a permutation)")
     order = np.ascontiguousarray(order, np.int32)
     rn = np.ascontiguousarray(roi_nodes, np.int64)
     wave = np.empty(n, np.int32)
@@ -135,12 +137,15 @@ def plan_waves(box: np.ndarray, order: np.ndarray, X: int, Y: int, roi_nodes: np
     if n == 0:
         return []
     box = np.ascontiguousarray(box.reshape(-1), np.int32)
+    n_nets = len(box) // 4
     order = np.ascontiguousarray(order, np.int32)
     rn = np.ascontiguousarray(roi_nodes, np.int64)
-    wave = np.empty(n, np.int32)
+    if len(rn) != n_nets:
+        raise ValueError("roi_nodes and box describe different net counts")
+    wave = np.full(n_nets, -1, np.int32)
     sched = np.empty(n, np.int32)
-    nw = lib.orx_plan_waves(box.ctypes.data, order.ctypes.data, n, X, Y, rn.ctypes.data, wave.ctypes.data,
-                            sched.ctypes.data)
+    nw = lib.orx_plan_waves(box.ctypes.data, n_nets, order.ctypes.data, n, X, Y, rn.ctypes.data,
+                            wave.ctypes.data, sched.ctypes.data)
     if nw < 0:
         raise ValueError(f"orx_plan_waves rejected its input ({nw})")
     ws = wave[sched]

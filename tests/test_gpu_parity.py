@@ -217,3 +217,13 @@ def test_nets_without_a_path_commit_nothing(device):
     _assert_same_state(dev, orc)
     st = dev.status()
     assert st[0] == 1 and st[1] == 1 and sd.failed == 2
+
+
+def test_hotset_mode_bit_exact(device):
+    """Hotset mode: offenders and unrouted nets rerouted (from the device's
+    per-net edge and node overuse), every net after a stalled iteration;
+    twelve iterations against the oracle's identical policy."""
+    nl = uneven(96, 80, 8, 300, seed=11)
+    cfg = RouterConfig(iterations=12, hotset=True, delta=16.0, block=1024, lds_nodes=16384)
+    dev, orc = _run_both(nl, cfg, 12, "batched", device)
+    assert min(s.routed for s in dev.history[1:]) < nl.n

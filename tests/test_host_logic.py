@@ -145,3 +145,31 @@ def test_device_built_base_costs_equal_host_array(dims):
     via = b[:, [0, 3]][np.isfinite(b[:, [0, 3]])]
     mv = m[:, [0, 3]][np.isfinite(m[:, [0, 3]])]
     assert (via.size == 0 and mv.size == 0) or mv.min() == via.min()
+
+
+def test_hotset_filter_keeps_offenders_in_order():
+    from orthoroute_amd.router import hotset_filter
+    order = np.array([4, 2, 0, 3, 1, 5], np.int32)
+    edge_over = np.array([0, 2, 0, 0, 0, 1])
+    node_over = np.array([0, 0, 0, 1, 0, 0])
+    hops = np.array([5, 7, 0, 9, 4, 3])          # net 2 has no path
+    assert hotset_filter(order, edge_over, node_over, hops).tolist() == [2, 3, 1, 5]
+    assert hotset_filter(order, np.zeros(6), np.zeros(6), np.ones(6)).tolist() == []
+
+
+def test_native_planner_on_a_subset_order():
+    """A hotset routes a subset of the nets: the planner's waves over that
+    subset equal the reference colouring of the same subset, and a net id
+    outside the netlist is rejected."""
+    from orthoroute_amd.waves import color_waves, group_waves, plan_waves
+    nl = uneven(96, 80, 6, 300, seed=2)
+    rng = np.random.default_rng(5)
+    order = rng.permutation(nl.n)[:170].astype(np.int32)
+    R = nl.roi_nodes()
+    got = plan_waves(nl.box, order, nl.spec.X, nl.spec.Y, R)
+    want = group_waves(color_waves(nl.box, order, nl.spec.X, nl.spec.Y), order, R)
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert a.tolist() == b.tolist()
+    with pytest.raises(ValueError):
+        plan_waves(nl.box, np.array([0, nl.n], np.int32), nl.spec.X, nl.spec.Y, R)

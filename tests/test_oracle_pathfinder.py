@@ -101,3 +101,27 @@ def test_wave_parallel_oracle_equals_sequential(threads):
             assert np.array_equal(x, y)
         assert np.array_equal(a.occ, b.occ)
         assert np.array_equal(a.hist.view(np.uint32), b.hist.view(np.uint32))
+
+
+def test_hotset_reroutes_offenders_and_converges():
+    """Hotset mode (offenders and unrouted nets; every net after an
+    iteration that did not lower the overuse) on 64x64x6: zero overuse
+    within 30 iterations, fewer reroutes than rerouting every net, and a
+    net outside the hotset keeps its path untouched."""
+    nl = plain_64()
+    o = OraclePathFinder(nl, RouterConfig(iterations=30, hotset=True))
+    routed, paths_before = [], None
+    for it in range(30):
+        order = o.order_for(it + 1)
+        if it > 0 and 0 < len(order) < nl.n:
+            paths_before = (set(range(nl.n)) - set(order.tolist()), [p.copy() for p in o.paths()])
+        d = o.run_iteration(order)
+        if paths_before is not None and it > 0 and 0 < len(order) < nl.n:
+            keep, before = paths_before
+            for k in keep:
+                assert np.array_equal(o.paths()[k], before[k])
+        routed.append(d["routed"])
+        if d["edge_over"] + d["node_over"] == 0:
+            break
+    assert d["edge_over"] + d["node_over"] == 0
+    assert sum(routed) < nl.n * len(routed)
