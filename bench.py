@@ -291,6 +291,54 @@ def convergence_bench(nl, cfg, max_iterations: int) -> dict:
     return out
 
 
+def roi_bench(n: int = 32, samples: int = 4, with_cpu: bool = True) -> dict:
+    """Device BFS regions of interest (extract_roi_bfs) on the 400x400x16
+    lattice: pins on layer 0 at offset (30, 20), radius 40; ms per
+    extraction including the host read of its size, and the oracle (numpy
+    restatement, one core) on the first ``samples`` pairs, which must give
+    the same ROIs."""
+    import numpy as np
+    import torch
+    from orthoroute_amd.lattice import LatticeSpec
+    from orthoroute_amd.roi import DeviceRoiBfs
+    from orthoroute_amd.solver import HipDijkstra
+    spec = LatticeSpec(400, 400, 16)
+    ip, ix, _, _ = spec.to_csr()
+    solver = HipDijkstra(indptr=ip, indices=ix)
+    r = DeviceRoiBfs(solver)
+    rng = np.random.default_rng(2)
+    pairs = []
+    for _ in range(n):
+        x, y = int(rng.integers(0, 370)), int(rng.integers(0, 380))
+        pairs.append((int(spec.node(x, y, 0)), int(spec.node(x + 30, y + 20, 0))))
+    r.extract_roi_bfs(*pairs[0], initial_radius=40)          # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sizes = []
+    for s_, d_ in pairs:
+        roi, _ = r.extract_roi_bfs(s_, d_, initial_radius=40)
+        sizes.append(int(roi.numel()))
+    dt = (time.perf_counter() - t0) / n
+    out = {"lattice": [400, 400, 16], "extractions": n, "ms_per_extraction": dt * 1e3,
+           "mean_roi_nodes": float(np.mean(sizes)), "depth_last": r.last["depth"]}
+    if with_cpu:
+        from oracle.oracle import roi_bfs
+        same = True
+        t0 = time.perf_counter()
+        for s_, d_ in pairs[:samples]:
+            oroi, _, _ = roi_bfs(ip, ix, spec.N, s_, d_, initial_radius=40)
+            roi, _ = r.extract_roi_bfs(s_, d_, initial_radius=40)
+            same &= bool(np.array_equal(roi.cpu().numpy(), oroi))
+        cpu = (time.perf_counter() - t0) / samples
+        out["cpu_baseline"] = {"ms_per_extraction": cpu * 1e3, "cores": 1, "kind": "port",
+                               "sample": f"the first {samples} pairs, numpy restatement (includes one device "
+                                         f"extraction each for the comparison)"}
+        out["same_as_oracle"] = same
+    del r, solver
+    torch.cuda.empty_cache()
+    return out
+
+
 def _traffic():
     """HBM bytes per route_wave_kernel dispatch from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
@@ -318,6 +366,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-iterations", type=int, default=50)
     ap.add_argument("--serve-seed", type=int, default=4)
+    ap.add_argument("--no-roi", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--converge-iterations", type=int, default=30)
     ap.add_argument("--no-portal", action="store_true")
@@ -428,6 +477,12 @@ def main() -> int:
                                 time_limit=300.0)
         torch.cuda.empty_cache()
 
+    roi = None
+    if not args.no_roi:
+        _log("device BFS regions of interest on 400x400x16")
+        roi = roi_bench(with_cpu=not args.no_cpu)
+        _log(f"roi: {roi['ms_per_extraction']:.2f} ms per extraction, {roi['mean_roi_nodes']:.0f} nodes")
+
     converge = None
     if not args.no_converge:
         _log(f"convergence: up to {args.converge_iterations} iterations, reroute-all and hotset")
@@ -486,6 +541,7 @@ def main() -> int:
         "serving": serving,
         "portal_nets": portal,
         "convergence": converge,
+        "roi_bfs": roi,
     }
     print(json.dumps(out))
     return 0

@@ -301,6 +301,27 @@ int orx_sssp_csr(const orx_csr* g, const float* d_costs, int64_t R, const int32_
                  uint64_t* d_key, uint32_t* d_scratch, int64_t scratch_words, int32_t* d_best,
                  const orx_route_cfg* cfg, orx_stream_t stream);
 
+/* Bidirectional BFS region of interest on the device
+ * (ROIExtractor.extract_roi_bfs UP:2257-2384; replaces its Python BFS and
+ * the per-net N-sized global_to_roi allocation, UP:2381-2382).  The caller
+ * passes the reference's derived limits: max_depth = min(initial_radius +
+ * 2*stagnation_bonus, 80), halo = min(16, 2 + floor(max(0, bonus)/0.6)),
+ * max_nodes (max_roi_nodes, 50 000), and d_keep[n_keep] = src, dst, then the
+ * portal seeds not yet listed.  d_blocked[E] (or NULL) skips arcs.
+ * Buffers: d_marks[ceil(N/16)] zero on entry (left zero), d_queue[4*qcap],
+ * d_blocks[ceil(N/4096)], d_roi[cap], d_g2r[N] -1 outside earlier ROIs.
+ * On return (stream order) d_roi[0 .. d_out[0]) is the ROI, ascending node
+ * ids then the appended must-keep nodes, cut to max_nodes as the reference
+ * does; d_g2r maps those nodes; d_out[1] depth, [2] meeting depth (-1),
+ * [3] seen nodes, [4] 0 or an overflow flag (queue 1, list 2), [5] seen
+ * nodes counted by the compaction.  orx_roi_release resets their d_g2r
+ * entries to -1; call it before the next orx_roi_bfs on the same buffers. */
+int orx_roi_bfs(const orx_csr* g, const uint8_t* d_blocked, int64_t src, int64_t dst, int32_t max_depth,
+                int32_t halo, int64_t max_nodes, const int32_t* d_keep, int32_t n_keep, uint32_t* d_marks,
+                int32_t* d_queue, int64_t qcap, uint32_t* d_blocks, int32_t* d_roi, int64_t cap, int32_t* d_g2r,
+                int64_t* d_out, orx_stream_t stream);
+int orx_roi_release(const int32_t* d_roi, const int64_t* d_out, int64_t cap, int32_t* d_g2r, orx_stream_t stream);
+
 /* Parent walk of a CSR query result (backtrace_parent CD:5857-5900,
  * backtrace_paths CD:1845-1950) from the target in d_target[0] (device, e.g.
  * the d_best of orx_sssp_csr): writes the global path, root first, into

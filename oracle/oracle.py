@@ -359,3 +359,71 @@ def py_box_dijkstra(spec, wcost: np.ndarray, box, src: int):
         par = -1 if sl == ROOT_SLOT else int(nb[v, sl])
         out[v] = (float(d), hp, par)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Bidirectional BFS region of interest (ROIExtractor.extract_roi_bfs,
+# unified_pathfinder.py:2257-2384), restated over whole BFS levels
+# ---------------------------------------------------------------------------
+def roi_bfs(indptr, indices, N: int, src: int, dst: int, initial_radius: int = 40, stagnation_bonus: float = 0.0,
+            portal_seeds=None, max_nodes: int = 50_000, blocked=None):
+    """(roi_nodes, global_to_roi) of extract_roi_bfs: level-synchronous
+    waves from src (mark 1) then dst (mark 2) per depth; a node reached by
+    the other wave is marked 3 and the waves have met; stop at max_depth =
+    min(int(radius + 2*bonus), 80), at meeting depth + halo (halo =
+    min(16, 2 + int(max(0, bonus) / 0.6))), or once more than 1.5*max_nodes
+    nodes are seen.  ROI = seen nodes ascending, then the missing must-keep
+    nodes; beyond max_nodes, must-keep positions past the cut swap to the
+    front positions before the cut."""
+    indptr = np.asarray(indptr, np.int64)
+    indices = np.asarray(indices, np.int64)
+    seen = np.zeros(N, np.uint8)
+    seen[src] = 1
+    seen[dst] = 2
+    fr = {1: np.asarray([src], np.int64), 2: np.asarray([dst], np.int64)}
+    max_depth = min(int(initial_radius + stagnation_bonus * 2.0), 80)
+    halo = min(16, 2 + int(max(0.0, stagnation_bonus) / 0.6))
+    depth, meet, met = 0, None, False
+    while depth < max_depth and (len(fr[1]) or len(fr[2])):
+        for mark in (1, 2):
+            u = fr[mark]
+            if len(u) == 0:
+                continue
+            starts, ends = indptr[u], indptr[u + 1]
+            arcs = np.concatenate([np.arange(a, b) for a, b in zip(starts, ends)]) if len(u) else np.zeros(0, np.int64)
+            if blocked is not None and len(arcs):
+                arcs = arcs[~np.asarray(blocked, bool)[arcs]]
+            v = np.unique(indices[arcs])
+            old = seen[v]
+            new = v[old == 0]
+            if np.any((old != 0) & (old != mark)):
+                met = True
+            seen[v[(old != 0) & (old != mark)]] = 3
+            seen[new] = mark
+            fr[mark] = new
+        depth += 1
+        if met and meet is None:
+            meet = depth
+        if meet is not None and depth >= meet + halo:
+            break
+        if int((seen > 0).sum()) > max_nodes * 1.5:
+            break
+    roi = np.flatnonzero(seen > 0).astype(np.int64)
+    keep = [src, dst]
+    for node, _ in (portal_seeds or []):
+        if node not in keep:
+            keep.append(node)
+    inset = set(roi.tolist())
+    missing = [n for n in keep if n not in inset]
+    if missing:
+        roi = np.append(roi, missing)
+    if roi.size > max_nodes:
+        kset = set(keep)
+        kept = [i for i, n in enumerate(roi.tolist()) if n in kset]
+        for swap_pos, k in enumerate(kept):
+            if k >= max_nodes:
+                roi[k], roi[swap_pos] = roi[swap_pos], roi[k]
+        roi = roi[:max_nodes]
+    g2r = np.full(N, -1, np.int32)
+    g2r[roi] = np.arange(len(roi), dtype=np.int32)
+    return roi.astype(np.int32), g2r, {"depth": depth, "meeting_depth": meet, "seen": int((seen > 0).sum())}

@@ -104,6 +104,9 @@ SIGNATURES = [
                                 P, C.c_int32, P, P, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg),
                                 C.c_int32, P]),
     ("orx_plan_iteration", C.c_int64, [P, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int64]),
+    ("orx_roi_bfs", C.c_int, [C.POINTER(OrxCsr), P, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int64, P,
+                              C.c_int32, P, P, C.c_int64, P, P, C.c_int64, P, P, P]),
+    ("orx_roi_release", C.c_int, [P, P, C.c_int64, P, P]),
     ("orx_plan_waves", C.c_int64, [P, C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P]),
     ("orx_net_overuse", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxNets), C.c_int32, C.c_int32, P, P, P, P]),
     ("orx_apply_paths", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), P, P, P, C.c_int32,

@@ -24,7 +24,7 @@ BUILD_DIR = REPO_DIR / "_build"
 LIB_PATH = PKG_DIR / "liborx_hip.so"
 ORACLE_DIR = REPO_DIR / "oracle"
 
-SOURCES = ["orx_capi.hip", "orx_lattice.hip", "orx_csr.hip", "orx_grid.hip", "orx_plan.cpp"]
+SOURCES = ["orx_capi.hip", "orx_lattice.hip", "orx_csr.hip", "orx_roi.hip", "orx_plan.cpp"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 HIPCC_FLAGS = [
@@ -58,7 +58,10 @@ def build_hip(verbose: bool = False) -> Path:
     """Compile every csrc/*.hip for gfx950 and link liborx_hip.so in-tree."""
     hipcc = _hipcc()
     BUILD_DIR.mkdir(exist_ok=True)
-    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    srcs = [CSRC / s for s in SOURCES]
+    missing = [str(s) for s in srcs if not s.exists()]
+    if missing:
+        raise FileNotFoundError(f"HIP sources missing: {missing}")
     objs = [BUILD_DIR / (s.stem + ".o") for s in srcs]
 
     def compile_one(pair):

@@ -43,6 +43,16 @@ def _host(a) -> np.ndarray:
     return np.asarray(a)
 
 
+def _in_roi(global_to_roi, nodes) -> np.ndarray:
+    """Membership of ``nodes`` in an ROI map; a device map is gathered at
+    those nodes only (no copy of the N-sized map)."""
+    idx = np.asarray(nodes, np.int64)
+    if isinstance(global_to_roi, torch.Tensor) and global_to_roi.is_cuda:
+        t = torch.from_numpy(idx).to(global_to_roi.device)
+        return (global_to_roi[t] >= 0).cpu().numpy()
+    return _host(global_to_roi)[idx] >= 0
+
+
 class HipDijkstra:
     """SSSP over a CSR graph on the GPU (generic form of the lattice kernels)."""
 
@@ -137,24 +147,25 @@ class HipDijkstra:
     # -- seam methods --------------------------------------------------------
     def find_path_roi_gpu(self, src: int, dst: int, costs, roi_nodes, global_to_roi,
                           node_penalty=None) -> Optional[List[int]]:
-        g2r = _host(global_to_roi)
-        if g2r[int(src)] < 0 or g2r[int(dst)] < 0:
+        if not _in_roi(global_to_roi, [int(src), int(dst)]).all():
             return None
         R = len(roi_nodes)
-        return self._search(costs, R, roi_nodes, g2r, node_penalty, None, [int(src)], None, [int(dst)], None,
-                            seed_pen=False)
+        return self._search(costs, R, roi_nodes, global_to_roi, node_penalty, None, [int(src)], None, [int(dst)],
+                            None, seed_pen=False)
 
     find_path_roi = find_path_roi_gpu
 
     def find_path_multisource_multisink(self, src_seeds: Sequence[Tuple[int, float]],
                                         dst_targets: Sequence[Tuple[int, float]], costs, roi_nodes,
                                         global_to_roi, node_penalty=None):
-        g2r = _host(global_to_roi)
-        seeds = [(int(n), float(c)) for n, c in src_seeds if g2r[int(n)] >= 0]
-        tgs = [(int(n), float(c)) for n, c in dst_targets if g2r[int(n)] >= 0]
+        src_seeds, dst_targets = list(src_seeds), list(dst_targets)
+        s_in = _in_roi(global_to_roi, [int(n) for n, _ in src_seeds]) if src_seeds else np.zeros(0, bool)
+        t_in = _in_roi(global_to_roi, [int(n) for n, _ in dst_targets]) if dst_targets else np.zeros(0, bool)
+        seeds = [(int(n), float(c)) for (n, c), ok in zip(src_seeds, s_in) if ok]
+        tgs = [(int(n), float(c)) for (n, c), ok in zip(dst_targets, t_in) if ok]
         if not seeds or not tgs:
             return None
-        path = self._search(costs, len(roi_nodes), roi_nodes, g2r, node_penalty, None,
+        path = self._search(costs, len(roi_nodes), roi_nodes, global_to_roi, node_penalty, None,
                             [n for n, _ in seeds], np.asarray([c for _, c in seeds], np.float32),
                             [n for n, _ in tgs], np.asarray([c for _, c in tgs], np.float32), seed_pen=True)
         if path is None:

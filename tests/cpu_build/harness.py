@@ -168,3 +168,39 @@ def sssp_box_host_threads(spec, wcost: np.ndarray, box, src: int, dst: int, work
     R = w * h * spec.L
     return (np.fromfile(d / "dist.out", np.float32, R), np.fromfile(d / "parent.out", np.int32, R),
             np.fromfile(d / "stats.out", np.int64, 8))
+
+
+ROI_DRIVER = HERE / "_out" / "roi_driver"
+
+
+def roi_bfs_host_threads(indptr, indices, N: int, src: int, dst: int, workdir: Path, initial_radius: int = 40,
+                         stagnation_bonus: float = 0.0, portal_seeds=None, max_nodes: int = 50_000, blocked=None,
+                         timeout: float = 600.0):
+    """One orx_roi_bfs + orx_roi_release through the host-thread build:
+    (roi, g2r, out[8], clean[2])."""
+    d = Path(workdir)
+    d.mkdir(parents=True, exist_ok=True)
+    keep = [src, dst]
+    for node, _ in (portal_seeds or []):
+        if node not in keep:
+            keep.append(node)
+    meta = {"N": N, "E": len(indices), "src": src, "dst": dst,
+            "max_depth": min(int(initial_radius + stagnation_bonus * 2.0), 80),
+            "halo": min(16, 2 + int(max(0.0, stagnation_bonus) / 0.6)), "max_nodes": max_nodes,
+            "n_keep": len(keep), "blocked": 0 if blocked is None else 1}
+    with open(d / "meta.txt", "w") as f:
+        for k, v in meta.items():
+            f.write(f"{k} {float(v)!r}\n")
+    np.ascontiguousarray(indptr, np.int32).tofile(d / "indptr")
+    ix = np.ascontiguousarray(indices, np.int32)
+    (ix if len(ix) else np.zeros(1, np.int32)).tofile(d / "indices")
+    np.asarray(keep, np.int32).tofile(d / "keep")
+    if blocked is not None:
+        np.asarray(blocked, np.uint8).tofile(d / "blocked")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
+    res = subprocess.run([str(ROI_DRIVER), str(d)], capture_output=True, text=True, timeout=timeout, env=env)
+    if res.returncode != 0:
+        raise RuntimeError(f"roi_driver failed ({res.returncode}):\n{res.stderr[-6000:]}")
+    out = np.fromfile(d / "out.out", np.int64)
+    return (np.fromfile(d / "roi.out", np.int32, int(out[0])), np.fromfile(d / "g2r.out", np.int32, N), out,
+            np.fromfile(d / "clean.out", np.int64))

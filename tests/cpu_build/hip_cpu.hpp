@@ -189,8 +189,18 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
 
 inline void __syncthreads() { hipcpu::g_blk->bar->arrive_and_wait(); }
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 template <typename T>
 T __shfl(T v, int src) { return hipcpu::shfl(v, (unsigned)src); }
+template <typename T>
+T __shfl_up(T v, unsigned d) {
+    uint64_t* ex = hipcpu::g_blk->exch.data() + 64 * hipcpu::wave();
+    ex[hipcpu::lane()] = hipcpu::to_bits(v);
+    hipcpu::wave_sync();
+    T r = hipcpu::lane() >= d ? hipcpu::from_bits<T>(ex[hipcpu::lane() - d]) : v;
+    hipcpu::wave_sync();
+    return r;
+}
 template <typename T>
 T __shfl_xor(T v, int m) {
     uint64_t* ex = hipcpu::g_blk->exch.data() + 64 * hipcpu::wave();
@@ -238,6 +248,8 @@ template <typename T, typename U>
 T atomicAdd(T* p, U v) { return std::atomic_ref<T>(*p).fetch_add((T)v); }
 template <typename T, typename U>
 T atomicMin(T* p, U v) { return __hip_atomic_fetch_min(p, (T)v, 0, 0); }
+template <typename T, typename U>
+T atomicOr(T* p, U v) { return std::atomic_ref<T>(*p).fetch_or((T)v); }
 template <typename T, typename U, typename V>
 T atomicCAS(T* p, U expected, V desired) {
     T cur = (T)expected;

@@ -185,3 +185,31 @@ def test_nets_without_a_path_commit_nothing(driver, tmp_path):
     assert np.array_equal(c.occ, o.occ)
     assert np.array_equal(c.node_use, o.node_use)
     assert np.array_equal(c.via_col, o.via_col)
+
+
+def test_roi_bfs_on_host_threads(driver, tmp_path):
+    """The device BFS region of interest (count / scan / scatter / finalize
+    kernels included) on host threads: ROI list, global_to_roi, depth and
+    meeting level equal the oracle's, marks and map left clean by the
+    release, every access in bounds.  Balls on a lattice, blocked arcs,
+    portal seeds and the truncation rule."""
+    from oracle.oracle import roi_bfs
+    from orthoroute_amd.lattice import LatticeSpec
+    from tests.cpu_build.harness import roi_bfs_host_threads
+    spec = LatticeSpec(40, 34, 4)
+    ip, ix, _, _ = spec.to_csr()
+    N = spec.N
+    rng = np.random.default_rng(9)
+    blocked = rng.random(len(ix)) < 0.15
+    cases = [dict(initial_radius=6), dict(initial_radius=40, stagnation_bonus=1.3),
+             dict(initial_radius=30, portal_seeds=[(5, 0.1), (N - 3, 0.2)], max_nodes=300),
+             dict(initial_radius=12, blocked=blocked), dict(initial_radius=80, max_nodes=100000)]
+    for i, kw in enumerate(cases):
+        src, dst = (int(v) for v in rng.choice(N, 2, replace=False))
+        roi, g2r, out, clean = roi_bfs_host_threads(ip, ix, N, src, dst, tmp_path / f"c{i}", **kw)
+        oroi, og2r, info = roi_bfs(ip, ix, N, src, dst, **kw)
+        assert np.array_equal(roi, oroi), i
+        assert np.array_equal(g2r, og2r), i
+        assert out[1] == info["depth"] and out[3] == info["seen"] == out[5], i
+        assert (out[2] if out[2] >= 0 else None) == info["meeting_depth"], i
+        assert out[4] == 0 and clean.tolist() == [0, 0], i

@@ -1,0 +1,103 @@
+"""Pins the oracle's bidirectional BFS region of interest
+(ROIExtractor.extract_roi_bfs, unified_pathfinder.py:2257-2384).
+
+  * On a lattice without blocked arcs the waves are BFS balls: the ROI is
+    every node within the final depth D of src or dst (scipy's unweighted
+    shortest paths), where the waves meet at depth ceil(dist(src, dst) / 2)
+    and stop `halo` levels later or at max_depth -- closed forms a dropped
+    level, an early or late meeting test or a wrong halo would break;
+  * blocked arcs are never followed;
+  * the reference's own test properties (T/unit/test_roi_extraction.py:
+    305-400): src, dst and portal seeds are always in the ROI and survive
+    truncation, and the result is deterministic; plus the truncation's exact
+    swap rule on a hand case.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import shortest_path
+
+from oracle.oracle import roi_bfs
+from orthoroute_amd.lattice import LatticeSpec
+
+
+def _graph(spec):
+    ip, ix, _, _ = spec.to_csr()
+    return ip, ix
+
+
+def _dist(ip, ix, N, s):
+    g = sp.csr_matrix((np.ones(len(ix)), ix, ip), shape=(N, N))
+    return shortest_path(g, unweighted=True, indices=s)
+
+
+@pytest.mark.parametrize("radius,bonus", [(3, 0.0), (40, 0.0), (40, 2.5), (6, 0.7)])
+def test_waves_are_bfs_balls_with_the_closed_form_depth(radius, bonus):
+    spec = LatticeSpec(30, 24, 4)
+    ip, ix = _graph(spec)
+    N = spec.N
+    rng = np.random.default_rng(int(radius * 10 + bonus * 7))
+    for _ in range(5):
+        src, dst = (int(v) for v in rng.choice(N, 2, replace=False))
+        roi, g2r, info = roi_bfs(ip, ix, N, src, dst, initial_radius=radius, stagnation_bonus=bonus)
+        ds, dd = _dist(ip, ix, N, src), _dist(ip, ix, N, dst)
+        L = int(ds[dst])
+        max_depth = min(int(radius + 2 * bonus), 80)
+        halo = min(16, 2 + int(max(0.0, bonus) / 0.6))
+        meet = (L + 1) // 2 if (L + 1) // 2 <= max_depth else None
+        if L == 1:
+            meet = 1
+        depth = max_depth if meet is None else min(max_depth, meet + halo)
+        want = np.flatnonzero((ds <= depth) | (dd <= depth))
+        assert info["meeting_depth"] == meet and info["depth"] == depth
+        assert np.array_equal(roi, want)
+        assert np.array_equal(g2r[roi], np.arange(len(roi)))
+        assert int((g2r >= 0).sum()) == len(roi)
+
+
+def test_blocked_arcs_are_not_followed():
+    spec = LatticeSpec(12, 10, 2)
+    ip, ix = _graph(spec)
+    N = spec.N
+    src, dst = int(spec.node(1, 1, 0)), int(spec.node(10, 8, 1))
+    blocked = np.zeros(len(ix), bool)
+    blocked[ix == dst] = True                   # nothing may enter dst
+    roi, _, _ = roi_bfs(ip, ix, N, src, dst, initial_radius=100, blocked=blocked)
+    ds = _dist(ip, ix, N, src)
+    assert dst in roi                           # seen at the start
+    # dst's wave still expands out of dst; src's wave covers every node it can reach
+    reach = np.flatnonzero(np.isfinite(ds))
+    assert set(reach.tolist()) <= set(roi.tolist()) | {dst}
+
+
+def test_must_keep_nodes_and_truncation_like_the_reference():
+    """Empty graph (no arcs), like T/unit/test_roi_extraction.py:364-383:
+    the ROI is src, dst and the portal seeds; with max_nodes 20 everything
+    survives.  Then a hand case of the swap rule on the path 0-1-...-29,
+    src 3, dst 25, max_nodes 10: after depth 4 the waves hold 0..7 and
+    21..29, 17 > 1.5 * 10 nodes, so the BFS stops; seed 35 is appended at
+    position 17.  The must-keep positions are 3, 12 (node 25) and 17:
+    position 12 swaps with 1 and 17 with 2, then the list is cut to 10."""
+    N = 400
+    ip = np.zeros(N + 1, np.int32)
+    ix = np.zeros(0, np.int32)
+    roi, g2r, _ = roi_bfs(ip, ix, N, 111, 288, initial_radius=50, portal_seeds=[(150, 0.5), (250, 0.5)],
+                          max_nodes=20)
+    assert roi.tolist() == [111, 288, 150, 250]   # sorted seen nodes, then the appended seeds
+    assert (g2r[[111, 288, 150, 250]] >= 0).all()
+    # the path graph 0-1-2-...-29 (nodes 30..39 isolated)
+    N = 40
+    ip = np.zeros(N + 1, np.int32)
+    nbrs = [[j for j in (i - 1, i + 1) if 0 <= j < 30] if i < 30 else [] for i in range(N)]
+    ip[1:] = np.cumsum([len(n) for n in nbrs])
+    ix = np.asarray([j for n in nbrs for j in n], np.int32)
+    roi, _, _ = roi_bfs(ip, ix, N, 3, 25, initial_radius=80, portal_seeds=[(35, 0.0)], max_nodes=10)
+    assert roi.tolist() == [0, 25, 35, 3, 4, 5, 6, 7, 21, 22]
+
+
+def test_deterministic():
+    spec = LatticeSpec(16, 16, 3)
+    ip, ix = _graph(spec)
+    a = roi_bfs(ip, ix, spec.N, 5, 700, initial_radius=7)
+    b = roi_bfs(ip, ix, spec.N, 5, 700, initial_radius=7)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
