@@ -125,3 +125,50 @@ def test_hotset_reroutes_offenders_and_converges():
             break
     assert d["edge_over"] + d["node_over"] == 0
     assert sum(routed) < nl.n * len(routed)
+
+
+def test_hotset_is_exactly_the_offender_set():
+    """The oracle's hotset order, iteration by iteration, against the
+    offender rule of _build_hotset (unified_pathfinder.py:9143-9171) worked
+    out here from the committed paths alone: a net is rerouted when a hop
+    of its path sits on an arc over capacity, a node of its path is over
+    capacity, or it has no path; the order is the iteration's base order
+    (net_order over per-net overuse recounted from the paths) restricted to
+    those nets; after an iteration that did not lower the overuse every net
+    is rerouted.  The same rule without its node term differs from the
+    oracle on this board, so a dropped term fails here."""
+    from orthoroute_amd.waves import net_order
+    nl = plain_64()
+    cfg = RouterConfig(iterations=14, hotset=True)
+    o = OraclePathFinder(nl, cfg)
+    nb = o.spec.neighbours()
+    totals, node_term_mattered, checked = [], False, 0
+    for it in range(1, 15):
+        order = o.order_for(it)
+        if it > 1:
+            occ, use = o.occ, o.node_use
+            over = np.zeros(nl.n, np.int64)
+            node_over = np.zeros(nl.n, np.int64)
+            hops = np.zeros(nl.n, np.int32)
+            for k, p in enumerate(o.paths()):
+                hops[k] = max(len(p) - 1, 0)
+                node_over[k] = int(np.maximum(use[p] - cfg.node_cap, 0).sum()) if len(p) else 0
+                for a, b in zip(p[:-1], p[1:]):
+                    s = int(np.nonzero(nb[a] == b)[0][0])
+                    over[k] += max(int(occ[4 * a + s]) - cfg.edge_cap, 0)
+            base = net_order(it, o.distance, over, hops)
+            stalled = len(totals) >= 2 and totals[-1] >= totals[-2]
+            if stalled:
+                want = base
+            else:
+                want = np.asarray([k for k in base if over[k] > 0 or node_over[k] > 0 or hops[k] == 0], np.int32)
+                edge_only = [k for k in base if over[k] > 0 or hops[k] == 0]
+                node_term_mattered |= edge_only != want.tolist()
+            assert np.array_equal(order, want), it
+            checked += 1
+        d = o.run_iteration(order)
+        totals.append(d["edge_over"] + d["node_over"])
+        if totals[-1] == 0:
+            break
+    assert checked >= 5
+    assert node_term_mattered
