@@ -375,6 +375,8 @@ def main() -> int:
     ap.add_argument("--portal-cpu-stride", type=int, default=4)
     ap.add_argument("--graph", action="store_true", help="capture each iteration's wave launches in a hipGraph")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-count", action="store_true",
+                    help="skip the counting replay (no algorithmic bytes): for counter passes over the timed kernel")
     ap.add_argument("--verify-lockstep", type=int, default=0, metavar="N",
                     help="only run N iterations of batched vs sequential order on --workload and print the result")
     ap.add_argument("--delta", type=float, default=None)
@@ -409,6 +411,7 @@ def main() -> int:
 
     _log(f"{nl.name}: {args.warmup} warm-up + {args.steps} timed iterations (delta {delta}, block {block})")
     dev = DevicePathFinder(nl, cfg)
+    dev.cfg.collect_stats = False
     first = None
     first_paths = None
     for _ in range(args.warmup):
@@ -419,29 +422,32 @@ def main() -> int:
     # the counts come from a replay of the same iterations with it on (every
     # configuration computes the same result, so the counts are those of the
     # timed work).
-    snap = dev.snapshot()
-    dev.cfg.collect_stats = False
+    snap = None if args.no_count else dev.snapshot()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     timed = [dev.run_iteration("batched") for _ in range(args.steps)]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     digest_timed = dev.digest()
-    dev.restore(snap)
-    dev.cfg.collect_stats = True
-    counted = [dev.run_iteration("batched") for _ in range(args.steps)]
-    if dev.digest() != digest_timed:
-        raise RuntimeError("the counting replay did not reproduce the timed iterations")
+    counted = None
+    if snap is not None:
+        dev.restore(snap)
+        dev.cfg.collect_stats = True
+        counted = [dev.run_iteration("batched") for _ in range(args.steps)]
+        if dev.digest() != digest_timed:
+            raise RuntimeError("the counting replay did not reproduce the timed iterations")
     if first is None:
         first = timed[0]
     wall = t1 - t0
     routed = sum(s.routed for s in timed)
-    relaxed = sum(s.relaxed for s in counted)
-    settled = sum(s.settled for s in counted)
     route_s = sum(s.route_ms for s in timed) / 1e3
     waves = sum(s.waves for s in timed)
-    alg_bytes = 12 * relaxed + 16 * settled
-    achieved = alg_bytes / route_s / 1e9
+    relaxed = settled = alg_bytes = achieved = None
+    if counted is not None:
+        relaxed = sum(s.relaxed for s in counted)
+        settled = sum(s.settled for s in counted)
+        alg_bytes = 12 * relaxed + 16 * settled
+        achieved = alg_bytes / route_s / 1e9
     footprint = dev.footprint_bytes()
     _log(f"batched: {wall / args.steps * 1e3:.1f} ms/iteration, overuse {timed[-1].overuse}")
     del dev
@@ -450,7 +456,8 @@ def main() -> int:
     plain = None
     if not args.no_plain:
         _log("plain one-net-per-launch iteration 1")
-        pdev = DevicePathFinder(nl, RouterConfig(iterations=1, delta=delta, block=block, lds_nodes=lds_nodes))
+        pdev = DevicePathFinder(nl, RouterConfig(iterations=1, delta=delta, block=block, lds_nodes=lds_nodes,
+                                                 collect_stats=False))
         pst = pdev.run_iteration("plain")
         same = None
         if first_paths is not None:
@@ -522,16 +529,16 @@ def main() -> int:
                    "roi_margin": 4, "iterations_timed": [timed[0].iteration, timed[-1].iteration],
                    "delta": delta, "block": block, "lds_nodes": lds_nodes, "graph": args.graph,
                    "parallelism": "1 GPU, disjoint-ROI waves"},
-        "gteps": relaxed / route_s / 1e9,
+        "gteps": None if relaxed is None else relaxed / route_s / 1e9,
         "waves_per_iteration": waves / args.steps,
         "overuse_last": timed[-1].overuse,
         "hbm_footprint_gb": footprint / 1e9,
         "plain_vs_batched": plain,
         "p99_job_latency_s": serving["p99_latency_s"] if serving else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "route_wave_kernel",
-                     "algorithmic_bytes_per_launch": alg_bytes / n_dispatch,
+                     "algorithmic_bytes_per_launch": None if alg_bytes is None else alg_bytes / n_dispatch,
                      "kernel_ms_per_launch": route_s * 1e3 / n_dispatch,
                      "launches_per_iteration": n_dispatch / args.steps,
                      "traffic_source": str(TRAFFIC_FILE.relative_to(REPO)) if traffic_src else None},
