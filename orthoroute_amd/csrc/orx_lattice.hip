@@ -1238,20 +1238,35 @@ __global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, i
 //   RELAX  : every workgroup relaxes a slice of the round's near queue;
 //   SPLIT1 : drop settled far entries, compact the live ones, min f;
 //   SPLIT2 : entries below the new threshold form the next near queue.
-// The last workgroup to finish a step (counted with an agent-scope atomic)
-// advances the state.  Launch sequences are replayed as a hipGraph by the
-// host.  Same keys, tie-break and near-far rule as box_sssp.
+// Each launch learns its step from the one before it, with no end-of-step
+// ticket: launch k reads the record rec[k & 1] that launch k - 1 wrote and
+// the counters launch k - 1 left (final once it ended), settles what the
+// record left open (a RELAX whose queue came out empty is a SPLIT1, or DONE),
+// and its workgroup 0 writes the record of launch k + 1 into rec[(k + 1) & 1],
+// which no workgroup of launch k reads.  Every counter a launch appends to
+// was cleared by an earlier launch that did not read it.  Launch sequences
+// are replayed as a hipGraph by the host (an even number of launches, so the
+// record parity carries over).  Same keys, tie-break and near-far rule as
+// box_sssp.
 // ---------------------------------------------------------------------------
 enum : uint32_t { GS_RELAX = 0, GS_SPLIT1 = 1, GS_SPLIT2 = 2, GS_DONE = 3 };
 
+// The step one launch executes, as far as the launch before it can tell.
+struct GridRec {
+    uint32_t state;     // RELAX / SPLIT2 / DONE (a RELAX may turn out a SPLIT1)
+    uint32_t r;         // near-far round: queue and bitmap parity, count slot
+    uint32_t phase;     // far-pile counter of the phase: fcnt[phase & 1]
+    uint32_t pad;
+    double T;           // threshold of the phase
+};
+
 struct GridCtl {
-    uint32_t state;
-    uint32_t r;
-    uint32_t qcnt[3];
-    uint32_t fcnt, fcnt2;
-    uint32_t blocks_done;
+    GridRec rec[2];
+    uint32_t qcnt[3];   // near-queue sizes, by round mod 3
+    uint32_t fcnt[2];   // far-pile size, by phase parity
+    uint32_t fcnt2;     // live far entries compacted by SPLIT1
+    uint32_t done, err;
     unsigned long long minf;
-    double T, Tn;
     unsigned long long rounds, phases, items, settled, relaxed;
 };
 
@@ -1279,6 +1294,7 @@ struct GridArgs {
     uint32_t rsrc, rdst;
     int32_t full;
     float delta;
+    uint32_t cap;       // entries of each queue array
     uint64_t* keys;
     uint32_t *bmn0, *bmn1, *bmf, *qA, *qB, *fA, *fB;
     GridCtl* ctl;
@@ -1296,11 +1312,13 @@ __global__ __launch_bounds__(256) void grid_init_kernel(GridArgs A) {
 __global__ void grid_seed_kernel(GridArgs A) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     GridCtl* c = A.ctl;
-    c->state = GS_RELAX; c->r = 0;
+    double T = (double)A.delta;
+    if (!(T > 0.0)) T = nextafter(0.0, (double)INFINITY);
+    c->rec[0].state = GS_RELAX; c->rec[0].r = 0; c->rec[0].phase = 0; c->rec[0].pad = 0; c->rec[0].T = T;
+    c->rec[1] = c->rec[0];
     c->qcnt[0] = 1; c->qcnt[1] = 0; c->qcnt[2] = 0;
-    c->fcnt = 0; c->fcnt2 = 0; c->blocks_done = 0;
-    c->minf = 0; c->T = (double)A.delta; c->Tn = 0.0;
-    if (!(c->T > 0.0)) c->T = nextafter(0.0, (double)INFINITY);
+    c->fcnt[0] = 0; c->fcnt[1] = 0; c->fcnt2 = 0; c->done = 0; c->err = 0;
+    c->minf = dbits((double)INFINITY);
     c->rounds = 0; c->phases = 0; c->items = 0; c->settled = 0; c->relaxed = 0;
     A.keys[A.rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
     A.qA[0] = A.rsrc;
@@ -1308,23 +1326,80 @@ __global__ void grid_seed_kernel(GridArgs A) {
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void grid_step_kernel(GridArgs A) {
+__global__ __launch_bounds__(BLOCK) void grid_step_kernel(GridArgs A, uint32_t par) {
     GridCtl* ctl = A.ctl;
-    const uint32_t state = ctl->state;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    // Every control field a step reads is loaded at once (one memory wait,
+    // not one per field; counts are picked after).
+    const GridRec rc = ctl->rec[par];
+    const uint32_t qc0 = gload_u32(&ctl->qcnt[0]), qc1 = gload_u32(&ctl->qcnt[1]), qc2 = gload_u32(&ctl->qcnt[2]);
+    const uint32_t fc0 = gload_u32(&ctl->fcnt[0]), fc1 = gload_u32(&ctl->fcnt[1]);
+    const uint32_t nf2 = gload_u32(&ctl->fcnt2);
+    const uint64_t kd = A.full ? ORX_KEY_INF : gkey_load(A.keys + A.rdst);
+    const unsigned long long mnb = __hip_atomic_load(&ctl->minf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t r = rc.r, j = rc.phase;
+    const double T = rc.T;
+    const uint32_t rq = r % 3;
+    const uint32_t n = rq == 0 ? qc0 : (rq == 1 ? qc1 : qc2);
+    const uint32_t nf = (j & 1) ? fc1 : fc0;
+    uint32_t state = rc.state;
+    if (state == GS_RELAX && n == 0) {
+        // the phase's bucket drained: stop (sink final, or nothing left
+        // far), else split the far pile
+        const bool sink = kd != ORX_KEY_INF && (double)kdist(kd) < T;
+        state = (sink || nf == 0) ? GS_DONE : GS_SPLIT1;
+    } else if (state == GS_SPLIT2 && nf2 == 0) {
+        state = GS_DONE;
+    }
+    // counts beyond the queue arrays would be a broken state: stop with an error
+    bool bad = false;
+    if ((state == GS_RELAX && n > A.cap) || (state == GS_SPLIT1 && nf > A.cap) || (state == GS_SPLIT2 && nf2 > A.cap)) {
+        bad = true;
+        state = GS_DONE;
+    }
+    double Tn = 0.0;
+    if (state == GS_SPLIT2) {
+        const double mn = __longlong_as_double((long long)mnb);
+        Tn = mn + (double)A.delta;
+        if (!(Tn > mn)) Tn = nextafter(mn, (double)INFINITY);
+    }
+    if (lead) {
+        // the next launch's record, and the counters it may append to
+        GridRec nx = rc;
+        if (state == GS_DONE) {
+            nx.state = GS_DONE;
+            ctl->done = 1;
+            if (bad) ctl->err = 1;
+        } else if (state == GS_RELAX) {
+            nx.r = r + 1;
+            ctl->qcnt[(r + 2) % 3] = 0;
+            ctl->fcnt2 = 0;
+            ctl->minf = dbits((double)INFINITY);
+            ctl->rounds += 1;
+            ctl->items += n;
+        } else if (state == GS_SPLIT1) {
+            nx.state = GS_SPLIT2;
+            ctl->phases += 1;
+        } else {  // SPLIT2: the next phase starts at the new threshold
+            nx.state = GS_RELAX;
+            nx.phase = j + 1;
+            nx.T = Tn;
+            ctl->fcnt[j & 1] = 0;
+        }
+        ctl->rec[par ^ 1u] = nx;
+    }
     if (state == GS_DONE) return;
     const Box& B = A.B;
     const Lat& g = A.g;
-    const uint32_t r = ctl->r;
-    const double T = ctl->T;
     const uint32_t stride = gridDim.x * BLOCK;
     const uint32_t first = blockIdx.x * BLOCK;
     if (state == GS_RELAX) {
-        const uint32_t n = gload_u32(&ctl->qcnt[r % 3]);
         uint32_t* cur = (r & 1) ? A.qB : A.qA;
         uint32_t* nxt = (r & 1) ? A.qA : A.qB;
         uint32_t* bmc = (r & 1) ? A.bmn1 : A.bmn0;
         uint32_t* bmx = (r & 1) ? A.bmn0 : A.bmn1;
         uint32_t* cnx = &ctl->qcnt[(r + 1) % 3];
+        uint32_t* cfar = &ctl->fcnt[j & 1];
         for (uint32_t base = first; base < n; base += stride) {
             const uint32_t i = base + threadIdx.x;
             const bool active = i < n;
@@ -1346,29 +1421,44 @@ __global__ __launch_bounds__(BLOCK) void grid_step_kernel(GridArgs A) {
             }
             const float du = kdist(ku);
             const uint32_t hu = khops(ku);
+            // The four arcs' atomics go out as batches (key minima, then
+            // queue bits, then one reservation per queue and wavefront), so
+            // a lane waits for device memory three times, not once per arc
+            // and operation.
+            float nd[4];
+            uint64_t nk[4], old[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                bool push_near = false, push_far = false;
-                const uint32_t v = nb[s];
-                if (v != UINT32_MAX && !isinf(w4[s])) {
-                    const float nd = __fadd_rn(du, w4[s]);
-                    const uint64_t nk = kpack(nd, hu + 1, 3u - (uint32_t)s);
-                    if (nk < gkey_load(A.keys + v)) {
-                        const uint64_t old = gkey_min(A.keys + v, nk);
-                        if (nk < old) {
-                            if ((double)nd < T) push_near = gbit_set(bmx, v) == 0;
-                            else push_far = gbit_set(A.bmf, v) == 0;
-                        }
-                    }
-                }
-                int at = wave_append(push_near, cnx);
-                if (push_near) nxt[at] = v;
-                at = wave_append(push_far, &ctl->fcnt);
-                if (push_far) A.fA[at] = v;
+                nd[s] = __fadd_rn(du, w4[s]);
+                nk[s] = kpack(nd[s], hu + 1, 3u - (uint32_t)s);
+                old[s] = 0;
+                if (nb[s] != UINT32_MAX && !isinf(w4[s])) old[s] = gkey_min(A.keys + nb[s], nk[s]);
+            }
+            bool to_near[4], to_far[4];
+            uint32_t was[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                to_near[s] = nk[s] < old[s] && (double)nd[s] < T;
+                to_far[s] = nk[s] < old[s] && !to_near[s];
+                was[s] = 1;
+                if (to_near[s]) was[s] = gbit_set(bmx, nb[s]);
+                else if (to_far[s]) was[s] = gbit_set(A.bmf, nb[s]);
+            }
+            uint32_t cn = 0, cf = 0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                cn += (to_near[s] && was[s] == 0) ? 1u : 0u;
+                cf += (to_far[s] && was[s] == 0) ? 1u : 0u;
+            }
+            uint32_t an = wave_append_n(cn, cnx);
+            uint32_t af = wave_append_n(cf, cfar);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                if (to_near[s] && was[s] == 0) nxt[an++] = nb[s];
+                if (to_far[s] && was[s] == 0) A.fA[af++] = nb[s];
             }
         }
     } else if (state == GS_SPLIT1) {
-        const uint32_t nf = gload_u32(&ctl->fcnt);
         double lmin = INFINITY;
         for (uint32_t base = first; base < nf; base += stride) {
             const uint32_t i = base + threadIdx.x;
@@ -1387,10 +1477,9 @@ __global__ __launch_bounds__(BLOCK) void grid_step_kernel(GridArgs A) {
         if (lane_id() == 0 && lmin < (double)INFINITY)
             __hip_atomic_fetch_min(&ctl->minf, dbits(lmin), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {  // GS_SPLIT2
-        const uint32_t nf2 = gload_u32(&ctl->fcnt2);
-        const double Tn = ctl->Tn;
         uint32_t* qr = (r & 1) ? A.qB : A.qA;
         uint32_t* bmr = (r & 1) ? A.bmn1 : A.bmn0;
+        uint32_t* cstay = &ctl->fcnt[(j + 1) & 1];
         for (uint32_t base = first; base < nf2; base += stride) {
             const uint32_t i = base + threadIdx.x;
             bool to_near = false, stay = false;
@@ -1403,56 +1492,8 @@ __global__ __launch_bounds__(BLOCK) void grid_step_kernel(GridArgs A) {
             }
             int at = wave_append(to_near, &ctl->qcnt[r % 3]);
             if (to_near) qr[at] = v;
-            at = wave_append(stay, &ctl->fcnt);
+            at = wave_append(stay, cstay);
             if (stay) A.fA[at] = v;
-        }
-    }
-    // the last workgroup to finish advances the state machine
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t prev = atomicAdd(&ctl->blocks_done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            ctl->blocks_done = 0;
-            if (state == GS_RELAX) {
-                ctl->rounds += 1;
-                ctl->items += gload_u32(&ctl->qcnt[r % 3]);
-                const uint32_t r1 = r + 1;
-                ctl->r = r1;
-                ctl->qcnt[(r1 + 1) % 3] = 0;
-                if (gload_u32(&ctl->qcnt[r1 % 3]) == 0) {
-                    bool done = false;
-                    if (!A.full) {
-                        const uint64_t kd = gkey_load(A.keys + A.rdst);
-                        done = kd != ORX_KEY_INF && (double)kdist(kd) < T;
-                    }
-                    if (done || gload_u32(&ctl->fcnt) == 0) {
-                        ctl->state = GS_DONE;
-                    } else {
-                        ctl->fcnt2 = 0;
-                        ctl->minf = dbits((double)INFINITY);
-                        ctl->phases += 1;
-                        ctl->state = GS_SPLIT1;
-                    }
-                }
-            } else if (state == GS_SPLIT1) {
-                if (gload_u32(&ctl->fcnt2) == 0) {
-                    ctl->state = GS_DONE;
-                } else {
-                    const double mn = __longlong_as_double((long long)__hip_atomic_load(
-                        &ctl->minf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                    double Tn = mn + (double)A.delta;
-                    if (!(Tn > mn)) Tn = nextafter(mn, (double)INFINITY);
-                    ctl->Tn = Tn;
-                    ctl->fcnt = 0;
-                    ctl->state = GS_SPLIT2;
-                }
-            } else {
-                ctl->T = ctl->Tn;
-                ctl->state = GS_RELAX;
-            }
-            __threadfence();
         }
     }
 }
@@ -1577,7 +1618,7 @@ __global__ void grid_stats_kernel(const GridCtl* c, unsigned long long* stats, i
         stats[0] += c->settled; stats[1] += c->relaxed; stats[2] += c->items;
         stats[3] += c->rounds; stats[4] += c->phases;
     }
-    *status = (!full && keys[rdst] == ORX_KEY_INF) ? ORX_NO_PATH : ORX_OK;
+    *status = c->err ? ORX_E_OVERFLOW : ((!full && keys[rdst] == ORX_KEY_INF) ? ORX_NO_PATH : ORX_OK);
 }
 
 // Host driver of the grid state machine: seed, replay a captured graph of
@@ -1598,6 +1639,7 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
     A.keys = ws->keys;
     A.qA = ws->queues; A.qB = ws->queues + wn; A.fA = ws->queues + 2 * wn; A.fB = ws->queues + 3 * wn;
     A.bmn0 = ws->bitmaps; A.bmn1 = ws->bitmaps + wn / 32; A.bmf = ws->bitmaps + 2 * (wn / 32);
+    A.cap = (uint32_t)std::min<int64_t>(wn, (int64_t)UINT32_MAX);
     int dev = 0, cus = 1;
     ORX_HIP_CHECK(hipGetDevice(&dev));
     ORX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1620,17 +1662,21 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
     }
     const int block = cfg->block;
     // Half a workgroup per CU: a step is short (a few thousand frontier
-    // nodes), so fewer workgroups mean fewer end-of-step tickets and a
-    // cheaper dispatch (1000x1000x12, 256 lanes: 128 groups 42 ms per query,
-    // 512 groups 55 ms, 1024 groups 76 ms).
+    // nodes), so fewer workgroups mean a cheaper dispatch (1000x1000x12, 256
+    // lanes: 64 or 128 groups 42 ms per query, 512 groups 55 ms, 1024 groups
+    // 76 ms, measured with an end-of-step ticket).
     const int grid = std::max(1, cus / 2);
     if (e == hipSuccess) e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
+        // kGridSteps is even: launch i reads control record i & 1, and every
+        // replay starts again at record 0
+        static_assert(kGridSteps % 2 == 0, "record parity must carry over between replays");
         for (int i = 0; i < kGridSteps; ++i) {
+            const uint32_t par = (uint32_t)(i & 1);
             switch (block) {
-            case 512: hipLaunchKernelGGL(grid_step_kernel<512>, dim3(grid), dim3(512), 0, ps, A); break;
-            case 1024: hipLaunchKernelGGL(grid_step_kernel<1024>, dim3(grid), dim3(1024), 0, ps, A); break;
-            default: hipLaunchKernelGGL(grid_step_kernel<256>, dim3(grid), dim3(256), 0, ps, A); break;
+            case 512: hipLaunchKernelGGL(grid_step_kernel<512>, dim3(grid), dim3(512), 0, ps, A, par); break;
+            case 1024: hipLaunchKernelGGL(grid_step_kernel<1024>, dim3(grid), dim3(1024), 0, ps, A, par); break;
+            default: hipLaunchKernelGGL(grid_step_kernel<256>, dim3(grid), dim3(256), 0, ps, A, par); break;
             }
         }
         hipError_t e2 = hipStreamEndCapture(ps, &graph);
@@ -1641,12 +1687,12 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
         // bounded wait: a search that has not finished after the time limit
         // reports an error instead of looping forever
         const auto t0 = std::chrono::steady_clock::now();
-        uint32_t state = GS_RELAX;
+        uint32_t done = 0;
         for (;;) {
             e = hipGraphLaunch(exec, ps);
-            if (e == hipSuccess) e = hipMemcpyAsync(&state, &ctl->state, sizeof(uint32_t), hipMemcpyDeviceToHost, ps);
+            if (e == hipSuccess) e = hipMemcpyAsync(&done, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, ps);
             if (e == hipSuccess) e = hipStreamSynchronize(ps);
-            if (e != hipSuccess || state == GS_DONE) break;
+            if (e != hipSuccess || done) break;
             if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300)) {
                 rc = orx_host::fail(ORX_E_OVERFLOW, "grid search did not finish within 300 s");
                 break;
@@ -1662,7 +1708,9 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(&h_status, d_status, sizeof(int32_t), hipMemcpyDeviceToHost, ps);
         if (e == hipSuccess) e = hipStreamSynchronize(ps);
-        if (e == hipSuccess) rc = h_status;
+        if (e == hipSuccess) rc = h_status == ORX_E_OVERFLOW
+                                      ? orx_host::fail(ORX_E_OVERFLOW, "grid search: a queue count exceeded its array")
+                                      : h_status;
     }
     if (e != hipSuccess && rc == ORX_OK) rc = orx_host::check_hip(e, "orx_sssp_box (grid)");
     if (exec) (void)hipGraphExecDestroy(exec);

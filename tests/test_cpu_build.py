@@ -106,9 +106,11 @@ def test_box_search_near_queue_spills_past_the_lds_ring(driver, tmp_path):
     np.testing.assert_array_equal(parent, op)
 
 
-def test_grid_search_on_host_threads(driver, tmp_path):
-    """The grid-wide state machine (captured step graph replayed until done)
-    on host threads: bit-exact with the oracle."""
+@pytest.mark.parametrize("delta", [1.0, 4.0, float("inf")])
+def test_grid_search_on_host_threads(driver, tmp_path, delta):
+    """The grid-wide state machine (captured step graph replayed until done;
+    each launch takes its step from the launch before): bit-exact with the
+    oracle over many threshold phases (delta 1), a few (4) and one (inf)."""
     from oracle.oracle import sssp_box as oracle_sssp_box
     from orthoroute_amd.lattice import LatticeSpec
     from tests.cpu_build.harness import sssp_box_host_threads
@@ -116,10 +118,46 @@ def test_grid_search_on_host_threads(driver, tmp_path):
     w = spec.random_edge_costs(9)
     box = (3, 2, 20, 16)
     src = int(spec.node(10, 10, 0))
-    dist, parent, stats = sssp_box_host_threads(spec, w, box, src, -1, tmp_path, grid=True, delta=4.0, block=256)
+    dist, parent, stats = sssp_box_host_threads(spec, w, box, src, -1, tmp_path, grid=True, delta=delta, block=256)
     od, op, _, _, _ = oracle_sssp_box(spec, w, box, src)
     assert np.array_equal(dist.view(np.uint32), od.view(np.uint32))
     np.testing.assert_array_equal(parent, op)
+    if delta == 1.0:
+        assert stats[4] > 10          # threshold phases (splits) taken
+
+
+def test_grid_search_stops_at_the_sink_on_host_threads(driver, tmp_path):
+    """The grid search with a sink stops once the sink is final: its distance
+    and its parent chain equal the oracle's full search."""
+    from oracle.oracle import sssp_box as oracle_sssp_box
+    from orthoroute_amd.lattice import LatticeSpec
+    from tests.cpu_build.harness import sssp_box_host_threads
+    spec = LatticeSpec(26, 20, 3)
+    w = spec.random_edge_costs(5)
+    box = (1, 1, 24, 18)
+    src, dst = int(spec.node(3, 4, 0)), int(spec.node(20, 15, 1))
+    dist, parent, _ = sssp_box_host_threads(spec, w, box, src, dst, tmp_path, grid=True, delta=2.0, block=256)
+    od, op, _, _, _ = oracle_sssp_box(spec, w, box, src)
+    x0, y0, bw, bh = box
+
+    def local(n):
+        z, rem = divmod(int(n), spec.X * spec.Y)
+        y, x = divmod(rem, spec.X)
+        return (z * bh + (y - y0)) * bw + (x - x0)
+
+    rd = local(dst)
+    assert dist[rd].view(np.uint32) == od[rd].view(np.uint32)
+    chain, v = [], rd
+    while v >= 0 and len(chain) <= len(dist):
+        chain.append(v)
+        p = parent[v]
+        v = local(p) if p >= 0 else -1
+    ochain, v = [], rd
+    while v >= 0 and len(ochain) <= len(od):
+        ochain.append(v)
+        p = op[v]
+        v = local(p) if p >= 0 else -1
+    assert chain == ochain and chain[-1] == local(src)
 
 
 @pytest.mark.parametrize("mode,lds_nodes", [(2, 0), (1, 12288)])

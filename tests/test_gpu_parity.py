@@ -156,6 +156,37 @@ def test_sssp_grid_random_weights(device, delta):
         np.testing.assert_array_equal(parent.cpu().numpy(), op)
 
 
+@pytest.mark.parametrize("delta", [2.0, float("inf")])
+def test_sssp_grid_stops_at_the_sink(device, delta):
+    """Grid-wide search with a sink: it stops once the sink is final; the
+    sink's distance and parent chain equal the oracle's full search."""
+    spec = LatticeSpec(60, 50, 6)
+    w = spec.random_edge_costs(8)
+    lat = LatticeState(spec, device=device)
+    lat.wcost.copy_(torch.from_numpy(w))
+    box = (2, 3, 55, 44)
+    x0, y0, bw, bh = box
+
+    def local(n):
+        z, rem = divmod(int(n), spec.X * spec.Y)
+        y, x = divmod(rem, spec.X)
+        return (z * bh + (y - y0)) * bw + (x - x0)
+
+    for src, dst in ((spec.node(5, 6, 0), spec.node(50, 40, 3)), (spec.node(30, 20, 5), spec.node(31, 20, 5))):
+        print(f"[grid-sink] delta {delta} src {int(src)} dst {int(dst)}", flush=True)
+        dist, parent, rc = sssp_box(lat, box, int(src), int(dst), full=False, delta=delta, grid=True)
+        od, op, _, _, _ = oracle_sssp_box(spec, w, box, int(src))
+        dist, parent = dist.cpu().numpy(), parent.cpu().numpy()
+        rd = local(dst)
+        assert rc == 0 and dist[rd].view(np.uint32) == od[rd].view(np.uint32)
+        chain, ochain = [rd], [rd]
+        while parent[chain[-1]] >= 0 and len(chain) <= len(dist):
+            chain.append(local(parent[chain[-1]]))
+        while op[ochain[-1]] >= 0 and len(ochain) <= len(od):
+            ochain.append(local(op[ochain[-1]]))
+        assert chain == ochain and chain[-1] == local(src)
+
+
 @pytest.mark.parametrize("lds_nodes,block", [(0, 1024), (16384, 1024), (12288, 256)])
 def test_portal_nets_bit_exact(device, lds_nodes, block):
     """Multi-terminal nets (find_path_multisource_multisink semantics): every
