@@ -8,6 +8,7 @@ shape.  Results never depend on these parameters (every configuration
 computes the same canonical keys); only time does, and the tuner checks it.
 
     python -m orthoroute_amd.autotune uneven_800
+    python -m orthoroute_amd.autotune backplane_400 --deltas 16,256,1024,4096 --blocks 1024 --lds-nodes 16384
 """
 from __future__ import annotations
 
@@ -42,7 +43,7 @@ def save_tuned(nl, params: dict) -> None:
     TUNED_PATH.write_text(json.dumps(table, indent=1, sort_keys=True))
 
 
-def tune(nl, deltas=(4.0, 8.0, 16.0, 32.0), blocks=(256, 512, 1024), lds_nodes=(8192, 16384),
+def tune(nl, deltas=(4.0, 8.0, 16.0, 32.0, 64.0, 256.0, 1024.0), blocks=(256, 512, 1024), lds_nodes=(8192, 16384),
          iteration: int = 2, save: bool = True, log=print) -> dict:
     """Time every configuration on the same full iteration (``iteration``,
     default 2: the first congested one) of ``nl``.
@@ -85,6 +86,22 @@ def tune(nl, deltas=(4.0, 8.0, 16.0, 32.0), blocks=(256, 512, 1024), lds_nodes=(
 
 
 if __name__ == "__main__":
+    import argparse
+
     from . import netgen
-    name = sys.argv[1] if len(sys.argv) > 1 else "uneven_800"
-    print(json.dumps(tune(getattr(netgen, name)(), log=lambda m: print(m, flush=True))))
+    ap = argparse.ArgumentParser(description="time launch parameters on one PathFinder iteration of a workload")
+    ap.add_argument("workload", nargs="?", default="uneven_800")
+    ap.add_argument("--deltas", default=None, help="comma-separated bucket widths")
+    ap.add_argument("--blocks", default=None, help="comma-separated workgroup sizes")
+    ap.add_argument("--lds-nodes", default=None, help="comma-separated LDS key capacities")
+    ap.add_argument("--no-save", action="store_true")
+    a = ap.parse_args()
+    kw = {}
+    if a.deltas:
+        kw["deltas"] = tuple(float(v) for v in a.deltas.split(","))
+    if a.blocks:
+        kw["blocks"] = tuple(int(v) for v in a.blocks.split(","))
+    if a.lds_nodes:
+        kw["lds_nodes"] = tuple(int(v) for v in a.lds_nodes.split(","))
+    print(json.dumps(tune(getattr(netgen, a.workload)(), save=not a.no_save, log=lambda m: print(m, flush=True),
+                          **kw)))
