@@ -113,7 +113,8 @@ def verify_lockstep(nl, cfg, iterations: int) -> dict:
                 and ss.path_hops == sb.path_hops and seq.digest() == bat.digest())
         rows.append({"iteration": it + 1, "overuse": sb.overuse, "plain_ms": round(ss.seconds * 1e3, 1),
                      "batched_ms": round(sb.seconds * 1e3, 1), "bit_exact": same})
-        _log(f"lockstep {nl.name} iteration {it + 1}: overuse {sb.overuse}, bit-exact {same}")
+        _log(f"lockstep {nl.name} iteration {it + 1}: overuse {sb.overuse}, plain {ss.seconds * 1e3:.1f} ms, "
+             f"batched {sb.seconds * 1e3:.1f} ms, bit-exact {same}")
         if not same:
             break
     return {"workload": nl.name, "iterations": len(rows), "all_bit_exact": all(r["bit_exact"] for r in rows),
