@@ -58,7 +58,7 @@ def tune(nl, deltas=(4.0, 8.0, 16.0, 32.0, 64.0, 256.0, 1024.0), blocks=(256, 51
 
     from .router import DevicePathFinder, RouterConfig
 
-    base_cfg = RouterConfig(iterations=iteration, collect_stats=False, **DEFAULTS)
+    base_cfg = RouterConfig(iterations=iteration, collect_stats=False, adapt_delta=False, **DEFAULTS)
     job = DevicePathFinder(nl, base_cfg)
     for _ in range(iteration - 1):
         job.run_iteration()

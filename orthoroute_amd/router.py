@@ -21,6 +21,7 @@ Buffers are PyTorch-ROCm tensors; kernels are called through the C ABI
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 from dataclasses import dataclass, field
 
@@ -60,6 +61,8 @@ class RouterConfig:
     graph: bool = False               # capture each iteration's wave launches in one hipGraph
     ws_cap_nodes: int = 1 << 24       # search workspace bound: larger waves launch in parts
     hotset: bool = False              # reroute only offenders (hotset_filter) unless the overuse stalled
+    adapt_delta: bool = True          # widen delta x4 after an iteration of many thin threshold phases
+    delta_max: float = 1048576.0
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -421,6 +424,7 @@ class DevicePathFinder:
         self.astar = self.cfg.astar_bounds(self.spec.base_cost_minima() if base is None else base)
         self.iteration = 0
         self.max_groups = 0
+        self.delta_mult = 1.0     # adaptive widening of cfg.delta (adapt_delta)
         self.history: list[IterationStats] = []
         self._pending: _Pending | None = None
         self._staged_for = -1     # iteration whose order input is staged in _over_h / _hops_h
@@ -441,7 +445,7 @@ class DevicePathFinder:
         under other launch parameters)."""
         if self._pending is not None:
             raise RuntimeError("an iteration is in flight")
-        snap = {"iteration": self.iteration, "history": len(self.history)}
+        snap = {"iteration": self.iteration, "history": len(self.history), "delta_mult": self.delta_mult}
         with torch.cuda.stream(self.stream):
             for owner, names in self._STATE:
                 obj = getattr(self, owner)
@@ -458,13 +462,35 @@ class DevicePathFinder:
                 for n in names:
                     getattr(obj, n).copy_(snap[(owner, n)])
         self.iteration = snap["iteration"]
+        self.delta_mult = snap["delta_mult"]
         self._staged_for = -1
         del self.history[snap["history"]:]
+
+    @property
+    def delta(self) -> float:
+        """Bucket width of the next launch: cfg.delta, widened by adapt_delta."""
+        return float(self.cfg.delta) * self.delta_mult
 
     def route_cfg(self, flags: int = 0) -> _lib.OrxRouteCfg:
         c = self.cfg
         fl = flags | (_lib.ORX_F_STATS if c.collect_stats else 0)
-        return _lib.OrxRouteCfg(float(c.delta), int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr())
+        return _lib.OrxRouteCfg(self.delta, int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr())
+
+    def _adapt_delta(self, st: "IterationStats") -> None:
+        """Widen delta x4 (up to delta_max) after an iteration whose searches
+        ran more than 8 threshold phases per net with rounds under a quarter
+        of a workgroup: the weights have outgrown the bucket width (a
+        congested board late in the negotiation), so a narrow bucket only
+        yields thin bands of nodes per round.  Boards whose searches end in
+        their first bucket (A* toward the sink) never trigger it.  Results do
+        not depend on delta; only time does."""
+        c = self.cfg
+        nets = st.routed + st.failed
+        if not c.adapt_delta or nets == 0 or st.rounds == 0 or math.isinf(c.delta):
+            return
+        if st.phases > 8 * nets and 4 * st.items < st.rounds * c.block and \
+                float(c.delta) * self.delta_mult * 4.0 <= c.delta_max:
+            self.delta_mult *= 4.0
 
     def _stage_net_overuse(self) -> None:
         """Enqueue the per-net overuse / hop counts of the committed paths and
@@ -627,6 +653,7 @@ class DevicePathFinder:
         self.iteration = p.it
         self._staged_for = p.it + 1 if self.netlist.n and p.it < len(self.schedule) else -1
         self.history.append(st)
+        self._adapt_delta(st)
         return st
 
     @property
@@ -672,6 +699,7 @@ class DevicePathFinder:
             extra={"launches": 1, "deps": int(len(dep_ids))})
         self.iteration = it
         self.history.append(st)
+        self._adapt_delta(st)
         return st
 
     def run(self, iterations: int | None = None, mode: str = "batched") -> list[IterationStats]:

@@ -173,3 +173,32 @@ def test_native_planner_on_a_subset_order():
         assert a.tolist() == b.tolist()
     with pytest.raises(ValueError):
         plan_waves(nl.box, np.array([0, nl.n], np.int32), nl.spec.X, nl.spec.Y, R)
+
+
+def test_adaptive_delta_widens_only_after_many_thin_phases():
+    """Delta grows x4 after an iteration with more than 8 threshold phases
+    per net and rounds under a quarter of a workgroup (the congested case);
+    an iteration whose searches end in their first bucket leaves it, and it
+    stops at delta_max."""
+    from types import SimpleNamespace
+
+    from orthoroute_amd.router import DevicePathFinder, IterationStats, RouterConfig
+
+    def stats(phases, items, rounds, routed=100):
+        return IterationStats(iteration=1, pres_fac=0.5, routed=routed, failed=0, edge_over=0, edge_over_count=0,
+                              node_over=0, node_over_count=0, via_hops=0, barrels=0, items=items, rounds=rounds,
+                              phases=phases)
+
+    job = SimpleNamespace(cfg=RouterConfig(delta=16.0, block=1024), delta_mult=1.0)
+    DevicePathFinder._adapt_delta(job, stats(phases=5, items=60_000, rounds=1000))      # A*: first bucket
+    assert job.delta_mult == 1.0
+    DevicePathFinder._adapt_delta(job, stats(phases=2700, items=48_000, rounds=1000))   # thin phases
+    assert job.delta_mult == 4.0
+    DevicePathFinder._adapt_delta(job, stats(phases=2700, items=900_000, rounds=1000))  # full rounds
+    assert job.delta_mult == 4.0
+    for _ in range(20):
+        DevicePathFinder._adapt_delta(job, stats(phases=2700, items=48_000, rounds=1000))
+    assert 16.0 * job.delta_mult == 1048576.0
+    off = SimpleNamespace(cfg=RouterConfig(delta=16.0, block=1024, adapt_delta=False), delta_mult=1.0)
+    DevicePathFinder._adapt_delta(off, stats(phases=2700, items=48_000, rounds=1000))
+    assert off.delta_mult == 1.0
