@@ -242,8 +242,11 @@ def test_roi_bfs_on_host_threads(driver, tmp_path):
     cases = [dict(initial_radius=6), dict(initial_radius=40, stagnation_bonus=1.3),
              dict(initial_radius=30, portal_seeds=[(5, 0.1), (N - 3, 0.2)], max_nodes=300),
              dict(initial_radius=12, blocked=blocked), dict(initial_radius=80, max_nodes=100000)]
+    cases.append(dict(initial_radius=40, same=True))      # src == dst
     for i, kw in enumerate(cases):
         src, dst = (int(v) for v in rng.choice(N, 2, replace=False))
+        if kw.pop("same", False):
+            dst = src
         roi, g2r, out, clean = roi_bfs_host_threads(ip, ix, N, src, dst, tmp_path / f"c{i}", **kw)
         oroi, og2r, info = roi_bfs(ip, ix, N, src, dst, **kw)
         assert np.array_equal(roi, oroi), i

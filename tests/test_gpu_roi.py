@@ -1,8 +1,8 @@
 """GPU: the device BFS region of interest (extract_roi_bfs,
 unified_pathfinder.py:2257-2384) against the oracle, exact: ROI node list
 (order included), global_to_roi, depth, meeting level -- over balls,
-blocked arcs, portal seeds and truncation, with buffers reused across calls;
-and the device ROI handed straight to find_path_roi_gpu."""
+blocked arcs, portal seeds, truncation and src == dst, with buffers reused
+across calls; and the device ROI handed straight to find_path_roi_gpu."""
 import numpy as np
 import pytest
 import torch
@@ -36,8 +36,11 @@ def test_device_roi_equals_the_oracle(lattice_solver):
         if i % 4 == 1:
             kw["max_nodes"] = int(rng.choice([50, 400, 3000]))
         cases.append((i % 5 == 2, kw))
+    cases += [(False, {"initial_radius": 40}), (True, {"initial_radius": 3, "stagnation_bonus": 3.1})]
     for i, (use_blocked, kw) in enumerate(cases):
         src, dst = (int(v) for v in rng.choice(N, 2, replace=False))
+        if i >= 24:
+            dst = src                  # the last two: src == dst
         print(f"[roi] case {i} src {src} dst {dst} {kw} blocked={use_blocked}", flush=True)
         r = with_blocked if use_blocked else plain
         max_nodes = kw.pop("max_nodes", 50_000)

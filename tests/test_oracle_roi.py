@@ -101,3 +101,21 @@ def test_deterministic():
     a = roi_bfs(ip, ix, spec.N, 5, 700, initial_radius=7)
     b = roi_bfs(ip, ix, spec.N, 5, 700, initial_radius=7)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_source_equal_to_sink():
+    """src == dst: seen[src] = 1 then seen[dst] = 2 leaves mark 2, so the
+    dst wave meets the src wave's first level at depth 1 (UP:2277-2280,
+    2305-2330): the ROI is the ball of radius min(max_depth, 1 + halo)."""
+    spec = LatticeSpec(20, 16, 3)
+    ip, ix = _graph(spec)
+    N = spec.N
+    src = int(spec.node(9, 7, 1))
+    for radius, bonus in ((40, 0.0), (2, 0.0), (40, 3.1)):
+        roi, g2r, info = roi_bfs(ip, ix, N, src, src, initial_radius=radius, stagnation_bonus=bonus)
+        halo = min(16, 2 + int(max(0.0, bonus) / 0.6))
+        depth = min(min(int(radius + 2 * bonus), 80), 1 + halo)
+        ds = _dist(ip, ix, N, src)
+        assert info["meeting_depth"] == 1 and info["depth"] == depth
+        assert np.array_equal(roi, np.flatnonzero(ds <= depth))
+        assert g2r[src] >= 0
