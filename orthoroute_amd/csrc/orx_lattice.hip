@@ -1683,16 +1683,30 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
         e = e2;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    // Replays run two deep: replay k + 1 is queued before the host waits for
+    // replay k's done flag, so the device does not idle through the host's
+    // round trip (the replay queued after the end runs only no-op launches).
+    uint32_t* h_done = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h_done, 2 * sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    auto replay = [&](int slot) -> hipError_t {
+        hipError_t x = hipGraphLaunch(exec, ps);
+        if (x == hipSuccess) x = hipMemcpyAsync(h_done + slot, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, ps);
+        if (x == hipSuccess) x = hipEventRecord(ev[slot], ps);
+        return x;
+    };
+    if (e == hipSuccess) e = replay(0);
     if (e == hipSuccess) {
         // bounded wait: a search that has not finished after the time limit
         // reports an error instead of looping forever
         const auto t0 = std::chrono::steady_clock::now();
-        uint32_t done = 0;
-        for (;;) {
-            e = hipGraphLaunch(exec, ps);
-            if (e == hipSuccess) e = hipMemcpyAsync(&done, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, ps);
-            if (e == hipSuccess) e = hipStreamSynchronize(ps);
-            if (e != hipSuccess || done) break;
+        for (int k = 0;; ++k) {
+            const int cur = k & 1;
+            e = replay(cur ^ 1);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[cur]);
+            if (e != hipSuccess || h_done[cur]) break;
             if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300)) {
                 rc = orx_host::fail(ORX_E_OVERFLOW, "grid search did not finish within 300 s");
                 break;
@@ -1713,8 +1727,12 @@ int sssp_box_grid(const Lat& g, int32_t x0, int32_t y0, int32_t w, int32_t h, in
                                       : h_status;
     }
     if (e != hipSuccess && rc == ORX_OK) rc = orx_host::check_hip(e, "orx_sssp_box (grid)");
+    (void)hipStreamSynchronize(ps);   // no replay or copy in flight past this point
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (ev[0]) (void)hipEventDestroy(ev[0]);
+    if (ev[1]) (void)hipEventDestroy(ev[1]);
+    if (h_done) (void)hipHostFree(h_done);
     if (ctl) (void)hipFreeAsync(ctl, ps);
     if (d_status) (void)hipFreeAsync(d_status, ps);
     (void)hipStreamSynchronize(ps);

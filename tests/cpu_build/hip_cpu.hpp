@@ -284,6 +284,16 @@ inline hipError_t hipMallocAsync(void** p, size_t b, hipStream_t) { *p = malloc(
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { free(p); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t b, int, hipStream_t) { memcpy(d, s, b); return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+// streams run synchronously here, so events are always complete
+typedef void* hipEvent_t;
+constexpr unsigned hipEventDisableTiming = 2;
+constexpr unsigned hipHostMallocDefault = 0;
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = (hipEvent_t)0x1; return 0; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return 0; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return 0; }
+inline hipError_t hipHostMalloc(void** p, size_t b, unsigned) { *p = malloc(b); return *p ? 0 : 1; }
+inline hipError_t hipHostFree(void* p) { free(p); return 0; }
 // stream capture: launches on a capturing stream are recorded, and replayed
 // in order by hipGraphLaunch
 namespace hipcpu {
