@@ -5,7 +5,7 @@ Vectors restated from bbenchoff/OrthoRoute tests/test_gpu_fallback.py:
 104-117 (ROI-CSR prices the entered node: [7.0, 3.0]), 120-145 (seed pays
 its penalty: [1, 3]), 148-173 (fp32 absorption chain: [3, 2, 1, 0]),
 176-268 (random 6x6 grid, seed 20260727, cost equals float64 heapq,
-rel 1e-5).
+rel 1e-5); the data is in tests/golden/gpu_fallback_vectors.json.
 """
 import numpy as np
 import pytest
@@ -14,39 +14,42 @@ import torch
 from oracle.oracle import csr_path, sssp_csr
 from orthoroute_amd.lattice import LatticeSpec, in_rank_and_transpose
 from orthoroute_amd.solver import HipDijkstra, try_attach_hip
-from tests.test_reference_vectors import random_grid_cases, reference_cost, returned_cost
+from tests.test_reference_vectors import GOLDEN, f32, random_grid_cases, reference_cost, returned_cost
 
 pytestmark = pytest.mark.gpu
 
 
 def test_roi_prices_destination_node(device):
-    hip = HipDijkstra(indptr=np.array([0, 1, 2]), indices=np.array([1, 0]), device=device)
-    costs = np.array([2.0, 3.0], np.float32)
-    pen = np.array([0.0, 5.0], np.float32)
-    roi = np.array([0, 1], np.int32)
-    assert hip.find_path_roi_gpu(0, 1, costs, roi, roi, node_penalty=pen) == [0, 1]
-    # the ROI-local search pays 2 + 5 to enter node 1
-    res = hip.find_path_multisource_multisink([(0, 0.0)], [(1, 0.0)], costs, roi, roi, node_penalty=pen)
-    assert res[0] == [0, 1]
+    v = GOLDEN["roi_csr_prices_destination_node"]
+    hip = HipDijkstra(indptr=np.array(v["indptr"]), indices=np.array(v["indices"]), device=device)
+    costs, pen = f32(v["costs"]), f32(v["node_penalty"])
+    roi = np.array(v["roi_nodes"], np.int32)
+    for q in v["queries"]:
+        assert hip.find_path_roi_gpu(q["src"], q["dst"], costs, roi, roi, node_penalty=pen) == q["path"]
+        # the ROI-local search pays the entered node's penalty (2 + 5 into node 1)
+        res = hip.find_path_multisource_multisink([(q["src"], 0.0)], [(q["dst"], 0.0)], costs, roi, roi,
+                                                  node_penalty=pen)
+        assert res[0] == q["path"]
 
 
 def test_fullgraph_prices_source_seed_node(device):
-    hip = HipDijkstra(indptr=np.array([0, 1, 2, 2, 2]), indices=np.array([3, 3]), device=device)
+    v = GOLDEN["fullgraph_prices_source_seed_node"]
+    hip = HipDijkstra(indptr=np.array(v["indptr"]), indices=np.array(v["indices"]), device=device)
     path = hip.find_path_fullgraph_gpu_seeds(
-        costs=torch.tensor([1.0, 2.0], device=device), src_seeds=np.array([0, 1], np.int32),
-        dst_targets=np.array([3], np.int32), src_seed_costs=np.zeros(2, np.float32),
-        dst_target_costs=np.zeros(1, np.float32),
-        node_penalty=torch.tensor([100.0, 0.0, 0.0, 0.0], device=device))
-    assert path == [1, 3]
+        costs=torch.from_numpy(f32(v["costs"])).to(device), src_seeds=np.array(v["seeds"], np.int32),
+        dst_targets=np.array(v["targets"], np.int32), src_seed_costs=f32(v["seed_costs"]),
+        dst_target_costs=f32(v["target_costs"]), node_penalty=torch.from_numpy(f32(v["node_penalty"])).to(device))
+    assert path == v["path"]
 
 
 def test_fullgraph_high_cost_rounding_does_not_cycle_parents(device):
-    hip = HipDijkstra(indptr=np.array([0, 1, 3, 5, 6]), indices=np.array([1, 0, 2, 1, 3, 2]), device=device)
+    v = GOLDEN["fullgraph_high_cost_rounding"]
+    hip = HipDijkstra(indptr=np.array(v["indptr"]), indices=np.array(v["indices"]), device=device)
     path = hip.find_path_fullgraph_gpu_seeds(
-        costs=torch.full((6,), 0.4, device=device), src_seeds=np.array([3], np.int32),
-        dst_targets=np.array([0], np.int32), src_seed_costs=np.array([16_000_000.0], np.float32),
-        dst_target_costs=np.zeros(1, np.float32))
-    assert path == [3, 2, 1, 0]
+        costs=torch.from_numpy(f32(v["costs"])).to(device), src_seeds=np.array(v["seeds"], np.int32),
+        dst_targets=np.array(v["targets"], np.int32), src_seed_costs=f32(v["seed_costs"]),
+        dst_target_costs=f32(v["target_costs"]))
+    assert path == v["path"]
 
 
 def test_fullgraph_matches_reference_dijkstra_on_random_costs(device):
@@ -58,8 +61,9 @@ def test_fullgraph_matches_reference_dijkstra_on_random_costs(device):
             dst_targets=np.array([target], np.int32), src_seed_costs=np.zeros(1, np.float32),
             dst_target_costs=np.zeros(1, np.float32), node_penalty=torch.from_numpy(penalty).to(device))
         assert path and path[0] == source and path[-1] == target
+        rel = GOLDEN["random_6x6_grid"]["rel"]
         assert returned_cost(indptr, indices, path, costs, penalty) == pytest.approx(
-            reference_cost(indptr, indices, costs, penalty, source, target), rel=1e-5, abs=1e-5)
+            reference_cost(indptr, indices, costs, penalty, source, target), rel=rel, abs=rel)
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])

@@ -1,5 +1,6 @@
 """The reference's own known-answer vectors for this path, restated against
-the CPU oracle (and, in test_gpu_seam.py, against the HIP path).
+the CPU oracle (and, in test_gpu_seam.py, against the HIP path).  The
+vectors are data in tests/golden/gpu_fallback_vectors.json.
 
 Sources (bbenchoff/OrthoRoute):
   tests/test_gpu_fallback.py:104-117  ROI-CSR folds the entered node's penalty: [7.0, 3.0]
@@ -9,12 +10,20 @@ Sources (bbenchoff/OrthoRoute):
                                        path cost == float64 heapq cost (rel 1e-5)
 """
 import heapq
+import json
+from pathlib import Path
 
 import numpy as np
 import pytest
 
 from oracle.oracle import csr_path, sssp_csr
 from orthoroute_amd.lattice import in_rank_and_transpose
+
+GOLDEN = json.loads((Path(__file__).resolve().parent / "golden" / "gpu_fallback_vectors.json").read_text())
+
+
+def f32(v):
+    return np.asarray(v, np.float32)
 
 
 def _graph(indptr, indices):
@@ -26,40 +35,42 @@ def _graph(indptr, indices):
 def test_roi_csr_prices_destination_node():
     # 2 nodes, arcs 0->1 (cost 2) and 1->0 (cost 3); penalty of node 1 = 5.
     # The folded ROI weights are [2+5, 3+0] = [7, 3]; the path 0 -> 1 costs 7.
-    N, ip, ix, rk, iip, isrc = _graph([0, 1, 2], [1, 0])
-    costs = np.array([2.0, 3.0], np.float32)
-    pen = np.array([0.0, 5.0], np.float32)
-    rc, best, dist, parent = sssp_csr(N, ip, ix, costs, rk, iip, isrc, roi_nodes=np.array([0, 1]),
-                                      g2r=np.array([0, 1]), pen=pen, seeds=[0], targets=[1], seed_pen=False)
-    assert rc == 0 and best == 1
-    assert dist[1] == np.float32(7.0)
-    rc, best, dist, parent = sssp_csr(N, ip, ix, costs, rk, iip, isrc, roi_nodes=np.array([0, 1]),
-                                      g2r=np.array([0, 1]), pen=pen, seeds=[1], targets=[0], seed_pen=False)
-    assert dist[0] == np.float32(3.0)
+    v = GOLDEN["roi_csr_prices_destination_node"]
+    N, ip, ix, rk, iip, isrc = _graph(v["indptr"], v["indices"])
+    roi = np.asarray(v["roi_nodes"])
+    for q in v["queries"]:
+        rc, best, dist, parent = sssp_csr(N, ip, ix, f32(v["costs"]), rk, iip, isrc, roi_nodes=roi, g2r=roi,
+                                          pen=f32(v["node_penalty"]), seeds=[q["src"]], targets=[q["dst"]],
+                                          seed_pen=False)
+        assert rc == 0 and best == q["dst"]
+        assert dist[q["dst"]] == np.float32(q["dist"])
+        assert csr_path(best, parent) == q["path"]
 
 
 def test_fullgraph_prices_source_seed_node():
-    N, ip, ix, rk, iip, isrc = _graph([0, 1, 2, 2, 2], [3, 3])
+    v = GOLDEN["fullgraph_prices_source_seed_node"]
+    N, ip, ix, rk, iip, isrc = _graph(v["indptr"], v["indices"])
     rc, best, dist, parent = sssp_csr(
-        N, ip, ix, np.array([1.0, 2.0], np.float32), rk, iip, isrc,
-        pen=np.array([100.0, 0.0, 0.0, 0.0], np.float32), seeds=[0, 1], seed_cost=np.zeros(2, np.float32),
-        targets=[3], tg_cost=np.zeros(1, np.float32))
+        N, ip, ix, f32(v["costs"]), rk, iip, isrc, pen=f32(v["node_penalty"]), seeds=v["seeds"],
+        seed_cost=f32(v["seed_costs"]), targets=v["targets"], tg_cost=f32(v["target_costs"]))
     assert rc == 0
-    assert csr_path(best, parent) == [1, 3]
+    assert csr_path(best, parent) == v["path"]
 
 
 def test_fullgraph_high_cost_rounding_does_not_cycle_parents():
-    N, ip, ix, rk, iip, isrc = _graph([0, 1, 3, 5, 6], [1, 0, 2, 1, 3, 2])
+    v = GOLDEN["fullgraph_high_cost_rounding"]
+    N, ip, ix, rk, iip, isrc = _graph(v["indptr"], v["indices"])
     rc, best, dist, parent = sssp_csr(
-        N, ip, ix, np.full(6, 0.4, np.float32), rk, iip, isrc, seeds=[3],
-        seed_cost=np.array([16_000_000.0], np.float32), targets=[0], tg_cost=np.zeros(1, np.float32))
+        N, ip, ix, f32(v["costs"]), rk, iip, isrc, seeds=v["seeds"], seed_cost=f32(v["seed_costs"]),
+        targets=v["targets"], tg_cost=f32(v["target_costs"]))
     assert rc == 0
-    assert csr_path(best, parent) == [3, 2, 1, 0]
+    assert csr_path(best, parent) == v["path"]
 
 
 def _grid6():
-    width = height = 6
-    rows = [[] for _ in range(36)]
+    g = GOLDEN["random_6x6_grid"]
+    width, height = g["width"], g["height"]
+    rows = [[] for _ in range(width * height)]
     for y in range(height):
         for x in range(width):
             node = y * width + x
@@ -67,7 +78,7 @@ def _grid6():
                 nx, ny = x + dx, y + dy
                 if 0 <= nx < width and 0 <= ny < height:
                     rows[node].append(ny * width + nx)
-    indptr = np.zeros(37, np.int32)
+    indptr = np.zeros(width * height + 1, np.int32)
     for node, nbrs in enumerate(rows):
         indptr[node + 1] = indptr[node] + len(nbrs)
     indices = np.asarray([v for r in rows for v in r], np.int32)
@@ -106,14 +117,16 @@ def returned_cost(indptr, indices, path, costs, penalty):
 
 def random_grid_cases():
     """The 8 trials of test_gpu_fullgraph_matches_reference_dijkstra_on_random_costs."""
+    g = GOLDEN["random_6x6_grid"]
     indptr, indices = _grid6()
-    rng = np.random.default_rng(20260727)
+    n = g["width"] * g["height"]
+    rng = np.random.default_rng(g["seed"])
     cases = []
-    for _ in range(8):
-        costs = rng.uniform(0.05, 8.0, len(indices)).astype(np.float32)
-        penalty = rng.uniform(0.0, 3.0, 36).astype(np.float32)
-        penalty[rng.random(36) < 0.7] = 0.0
-        source, target = rng.choice(36, size=2, replace=False).astype(np.int32)
+    for _ in range(g["trials"]):
+        costs = rng.uniform(*g["cost_range"], len(indices)).astype(np.float32)
+        penalty = rng.uniform(*g["penalty_range"], n).astype(np.float32)
+        penalty[rng.random(n) < g["penalty_zero_fraction"]] = 0.0
+        source, target = rng.choice(n, size=2, replace=False).astype(np.int32)
         cases.append((costs, penalty, int(source), int(target)))
     return indptr, indices, cases
 
@@ -127,5 +140,6 @@ def test_random_6x6_grid_matches_float64_dijkstra():
                                           tg_cost=np.zeros(1, np.float32))
         path = csr_path(best, parent)
         assert path and path[0] == source and path[-1] == target
+        rel = GOLDEN["random_6x6_grid"]["rel"]
         assert returned_cost(indptr, indices, path, costs, penalty) == pytest.approx(
-            reference_cost(indptr, indices, costs, penalty, source, target), rel=1e-5, abs=1e-5)
+            reference_cost(indptr, indices, costs, penalty, source, target), rel=rel, abs=rel)
