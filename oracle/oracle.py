@@ -102,6 +102,9 @@ def load() -> C.CDLL:
     lib.orx_oracle_lattice_weights.restype = None
     lib.orx_oracle_lattice_weights.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, P, P, P, P, C.POINTER(OracleParams), P]
+    lib.orx_oracle_apply_path.restype = C.c_int
+    lib.orx_oracle_apply_path.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int64, C.c_int32, P,
+                                          P, P]
     lib.orx_oracle_net_overuse.restype = None
     lib.orx_oracle_net_overuse.argtypes = [
         C.c_int32, C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32, C.c_int32, P, P, P, P, P]
@@ -253,6 +256,31 @@ class OraclePathFinder:
     def state(self) -> dict:
         return {"hist": self.hist, "occ": self.occ, "node_hist": self.node_hist, "node_use": self.node_use,
                 "via_col": self.via_col}
+
+
+def apply_path(spec, path, sign: int, occ: np.ndarray, node_use: np.ndarray, via_col: np.ndarray) -> None:
+    """Commit (+1) or rip up (-1) one path in place (UP:10034-10046,
+    4635-4659, 4500-4512)."""
+    p = np.ascontiguousarray(path, np.int32)
+    rc = load().orx_oracle_apply_path(spec.X, spec.Y, spec.L, spec.hmask, _p(p), len(p), int(sign), _p(occ),
+                                      _p(node_use), _p(via_col))
+    if rc != 0:
+        raise ValueError(f"path is not a lattice walk ({rc})")
+
+
+def overuse(spec, occ, node_use, via_col, edge_cap: int = 1, node_cap: int = 1) -> dict:
+    """Iteration-end counts over canonical arcs (UP:1137-1190, 4919-4932)."""
+    out = np.zeros(6, np.int64)
+    load().orx_oracle_overuse(spec.X, spec.Y, spec.L, _p(occ), _p(node_use), _p(via_col), edge_cap, node_cap,
+                              _p(out))
+    return dict(zip(("edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops", "barrels"),
+                    (int(v) for v in out)))
+
+
+def update_history(base, hist, occ, node_hist, node_use, params: OracleParams) -> None:
+    """h = fl(h*decay) + min(gain*over, cap_mult*base), in place (UP:1226-1289)."""
+    load().orx_oracle_update_history(len(hist), _p(base), _p(hist), _p(occ), len(node_hist), _p(node_hist),
+                                     _p(node_use), C.byref(params))
 
 
 def lattice_weights(spec, base, hist, occ, node_hist, node_use, params: OracleParams) -> np.ndarray:

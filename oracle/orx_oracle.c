@@ -603,6 +603,14 @@ static int apply_path(const lat_t* g, const int32_t* path, int64_t len, int sign
     return 0;
 }
 
+/* One commit (+1) or rip-up (-1) of one path, as the driver applies it
+ * (exported for the accountant's known-answer checks). */
+int orx_oracle_apply_path(int32_t X, int32_t Y, int32_t L, uint32_t hmask, const int32_t* path, int64_t len,
+                          int32_t sign, int32_t* occ, int32_t* node_use, int32_t* via_col) {
+    lat_t g = { X, Y, L, hmask };
+    return apply_path(&g, path, len, sign, occ, node_use, via_col);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Iteration-end accounting: overuse (canonical arcs only, slots 2 and 3,    */
 /* unified_pathfinder.py:4919-4932), then history                           */
