@@ -202,3 +202,25 @@ def test_adaptive_delta_widens_only_after_many_thin_phases():
     off = SimpleNamespace(cfg=RouterConfig(delta=16.0, block=1024, adapt_delta=False), delta_mult=1.0)
     DevicePathFinder._adapt_delta(off, stats(phases=2700, items=48_000, rounds=1000))
     assert off.delta_mult == 1.0
+
+
+@pytest.mark.parametrize("X,Y,L", [(7, 5, 2), (9, 6, 4), (64, 64, 6)])
+def test_lattice_arc_counts(X, Y, L):
+    """Directed arc counts of the lattice CSR.  The two-layer case is the
+    reference's own known answer (tests/test_lattice.py:131-146: lateral
+    2*xs*(ys-1) + 2*ys*(xs-1), vias 2*xs*ys); with planar arcs on every layer
+    (the ORTHO_LATERAL_ALL form, unified_pathfinder.py:1691-1701) each V
+    layer adds 2*X*(Y-1), each H layer 2*Y*(X-1), each via level 2*X*Y.
+    64x64x6 gives 89,344 arcs."""
+    from orthoroute_amd.lattice import LatticeSpec
+    spec = LatticeSpec(X, Y, L)
+    ip, ix, _, _ = spec.to_csr()
+    z = np.repeat(np.arange(L), X * Y)
+    src = np.repeat(np.arange(spec.N), np.diff(ip))
+    via = (ix // (X * Y)) != z[src]
+    n_v = (L + 1) // 2                       # layers 0, 2, ... are vertical
+    n_h = L // 2
+    assert int((~via).sum()) == n_v * 2 * X * (Y - 1) + n_h * 2 * Y * (X - 1)
+    assert int(via.sum()) == 2 * X * Y * (L - 1)
+    if (X, Y, L) == (64, 64, 6):
+        assert len(ix) == 89_344
