@@ -132,7 +132,7 @@ def copy_probe(mib: int = 256, reps: int = 5) -> None:
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "reps": reps}))
 
 
-def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1) -> dict:
+def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, tiles: bool = True) -> dict:
     """Full queries on 1000x1000x12 (costs U[1,4) per physical edge, a seeded
     5% x10), endpoints uniform on layer 0, one query per launch sequence."""
     import numpy as np
@@ -153,7 +153,7 @@ def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1) -> 
     box = (0, 0, spec.X, spec.Y)
     # warm-up query (not timed)
     sssp_box(lat, box, int(pairs[0, 0]), int(pairs[0, 1]), full=True, delta=delta, block=block, ws=ws,
-             stats=stats, grid=True)
+             stats=stats, grid=not tiles, tiles=tiles)
     stats.zero_()
     torch.cuda.synchronize()
     times = []
@@ -161,7 +161,7 @@ def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1) -> 
     for q in range(n_queries):
         t = time.perf_counter()
         dist, parent, rc = sssp_box(lat, box, int(pairs[q, 0]), int(pairs[q, 1]), full=True, delta=delta,
-                                    block=block, ws=ws, stats=stats, grid=True)
+                                    block=block, ws=ws, stats=stats, grid=not tiles, tiles=tiles)
         times.append(time.perf_counter() - t)
         if q == 0:
             check = _check_path_cost(lat, dist, parent, int(pairs[q, 0]), int(pairs[q, 1]))
@@ -380,6 +380,8 @@ def main() -> int:
                     help="skip the counting replay (no algorithmic bytes): for counter passes over the timed kernel")
     ap.add_argument("--verify-lockstep", type=int, default=0, metavar="N",
                     help="only run N iterations of batched vs sequential order on --workload and print the result")
+    ap.add_argument("--engine", default="tiles", choices=("tiles", "wave"),
+                    help="search engine: tile-step (default) or one workgroup per net")
     ap.add_argument("--delta", type=float, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--lds-nodes", type=int, default=None)
@@ -408,7 +410,8 @@ def main() -> int:
     delta = args.delta if args.delta is not None else tuned["delta"]
     block = args.block if args.block is not None else tuned["block"]
     lds_nodes = args.lds_nodes if args.lds_nodes is not None else tuned["lds_nodes"]
-    cfg = RouterConfig(iterations=args.warmup + args.steps, delta=delta, block=block, lds_nodes=lds_nodes, graph=args.graph)
+    cfg = RouterConfig(iterations=args.warmup + args.steps, delta=delta, block=block, lds_nodes=lds_nodes,
+                       graph=args.graph, engine=args.engine)
 
     _log(f"{nl.name}: {args.warmup} warm-up + {args.steps} timed iterations (delta {delta}, block {block})")
     dev = DevicePathFinder(nl, cfg)
@@ -458,6 +461,7 @@ def main() -> int:
     if not args.no_plain:
         _log("plain one-net-per-launch iteration 1")
         pdev = DevicePathFinder(nl, RouterConfig(iterations=1, delta=delta, block=block, lds_nodes=lds_nodes,
+                                                 engine=args.engine,
                                                  collect_stats=False))
         pst = pdev.run_iteration("plain")
         same = None
@@ -475,7 +479,7 @@ def main() -> int:
     sssp = None
     if not args.no_sssp:
         _log(f"SSSP microbench: {args.sssp_queries} queries on 1000x1000x12")
-        sssp = sssp_microbench(args.sssp_queries, args.sssp_delta, 256)
+        sssp = sssp_microbench(args.sssp_queries, args.sssp_delta, 256, tiles=args.engine == "tiles")
         torch.cuda.empty_cache()
 
     serving = None

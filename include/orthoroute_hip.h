@@ -144,6 +144,8 @@ typedef struct orx_route_cfg {
                             /* above 2^18 ROI nodes)                         */
 #define ORX_F_SEED_PEN 4    /* CSR seeds pay their own node penalty (multi-  */
                             /* seed forms UP:2582-2583, CD:5589-5606)        */
+#define ORX_F_TILES 16      /* orx_sssp_box: tile-step engine (the whole box  */
+                            /* relaxed to its fixpoint, see orx_route_tiles)  */
                             /* (bits 16 and up are reserved)                 */
 
 /* stats[]: 0 settled nodes (label-setting count up to the sink; for a
@@ -195,6 +197,25 @@ int orx_route_waves(const orx_lattice* lat, const orx_params* p, const orx_nets*
                     const int32_t* d_nets, const int64_t* d_ws_off, int32_t n_waves,
                     const int32_t* h_wave_ptr, const int64_t* h_wave_max, const int64_t* h_wave_ws,
                     const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream);
+
+/* The routing step of one iteration on the tile-step engine: the same
+ * waves, offsets and results as orx_route_waves (the per-net loop of
+ * _route_all, UP:7949-8430, batched over disjoint-ROI waves), with each
+ * net's search (heap Dijkstra UP:2451-2548 / the wavefront and persistent
+ * searches CD:2800-3533, PK:260-516) run as tiles of 16 x ty cells x all
+ * layers (ty = min(16, 5120 / (16 L))), relaxed to a local fixpoint in LDS
+ * by line scans and exchanging boundary keys between step launches.  Every
+ * net's box is searched to its fixpoint (no early exit), so paths, costs
+ * and parents are the canonical ones.  Per wave: one fill, one prep (rip-up,
+ * seeds), h_wave_steps[i] step launches, one single-workgroup drain of
+ * whatever is still active, one post (backtrace, commit, re-price).
+ * h_wave_steps ~ the largest net's tile count along x plus along y.  The
+ * workspace needs ws_nodes >= 512 and >= every h_wave_ws[i]; its queue and
+ * bitmap arrays hold the tile lists, bits and step counters.  L <= 32. */
+int orx_route_tiles(const orx_lattice* lat, const orx_params* p, const orx_nets* nets, const int32_t* d_nets,
+                    const int64_t* d_ws_off, int32_t n_waves, const int32_t* h_wave_ptr, const int64_t* h_wave_ws,
+                    const int32_t* h_wave_steps, const orx_workspace* ws, const orx_route_cfg* cfg,
+                    orx_stream_t stream);
 
 /* Launch-sequence capture into a hipGraph (no reference counterpart: the
  * reference launches each CuPy kernel from Python, CD:2968-3043).

@@ -23,6 +23,7 @@ ORX_F_STATS = 1
 ORX_F_FULL = 2
 ORX_F_SEED_PEN = 4
 ORX_F_GRID = 8
+ORX_F_TILES = 16
 
 STATUS_NAMES = {
     ORX_OK: "ok",
@@ -95,6 +96,8 @@ SIGNATURES = [
     ("orx_route_wave", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
                                  C.c_int32, C.c_int64, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
     ("orx_route_waves", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
+                                  C.c_int32, P, P, P, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
+    ("orx_route_tiles", C.c_int, [C.POINTER(OrxLattice), C.POINTER(OrxParams), C.POINTER(OrxNets), P, P,
                                   C.c_int32, P, P, P, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
     ("orx_graph_begin", C.c_int, [C.POINTER(C.c_void_p)]),
     ("orx_graph_end", C.c_int, [P, C.POINTER(C.c_void_p)]),

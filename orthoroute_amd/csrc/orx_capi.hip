@@ -40,7 +40,9 @@ int orx_init(void) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return orx_host::fail(ORX_E_NODEVICE, std::string("device is ") + prop.gcnArchName +
                                                    ", this library is built for gfx950 only");
-    return orx_host::prepare_lattice_kernels();
+    int rc = orx_host::prepare_lattice_kernels();
+    if (!rc) rc = orx_host::prepare_tile_kernels();
+    return rc;
 }
 
 // Launch-sequence capture: a private non-blocking stream records every orx_*

@@ -13,6 +13,8 @@
 #include <stdio.h>
 #include <string>
 
+#include "../../include/orthoroute_hip.h"
+
 #pragma clang fp contract(off)
 
 // Dynamic LDS of a kernel.  The host-thread test build of the kernels
@@ -64,6 +66,19 @@ __device__ __forceinline__ uint32_t lane_id() {
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
+
+// Direct global -> LDS load of 16 bytes per lane (gfx950 global_load_lds_dwordx4):
+// lane l's bytes land at lds_wave_base + 16 * l, with no register staging.
+// lds_wave_base must be the same on every lane of the wave.  Completion is
+// awaited with lds_load_wait() before the barrier that publishes the data.
+// The host-thread test build provides its own versions of both.
+#ifndef ORX_HOST_THREAD_BUILD
+__device__ __forceinline__ void lds_load16(const void* g, void* lds_wave_base) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)g,
+                                     (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+}
+__device__ __forceinline__ void lds_load_wait() { __builtin_amdgcn_s_waitcnt(0); }
+#endif
 
 // Wavefront-aggregated append: every lane of the wave must call this (uniform
 // control flow); lanes with want=true get consecutive slots.  One LDS atomic
@@ -167,6 +182,11 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 int prepare_lattice_kernels();  // raise the dynamic-LDS limit of the search kernels once
+int prepare_tile_kernels();     // same for the tile engine
+// whole-box query on the tile engine (orx_sssp_box with ORX_F_TILES)
+int sssp_box_tiles(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int32_t h, int64_t src, int64_t dst,
+                   float* d_dist, int32_t* d_parent, const orx_workspace* ws, const orx_route_cfg* cfg,
+                   hipStream_t caller);
 }  // namespace orx_host
 
 #define ORX_HIP_CHECK(expr)                                              \

@@ -63,6 +63,7 @@ class RouterConfig:
     hotset: bool = False              # reroute only offenders (hotset_filter) unless the overuse stalled
     adapt_delta: bool = True          # widen delta x4 after an iteration of many thin threshold phases
     delta_max: float = 1048576.0
+    engine: str = "tiles"             # "tiles": tile-step search (orx_route_tiles); "wave": one workgroup per net
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -270,7 +271,7 @@ class Workspace:
         self.struct = None
 
     def ensure(self, nodes: int):
-        nodes = int((nodes + 63) // 64 * 64)
+        nodes = int((max(int(nodes), TILE_MIN_WS) + 63) // 64 * 64)
         if nodes <= self.nodes:
             return
         nodes = max(nodes, int(self.nodes * 1.25))
@@ -285,6 +286,27 @@ class Workspace:
 
     def nbytes(self) -> int:
         return self.nodes * (8 + 16) + 3 * self.nodes // 8
+
+
+TILE_X = 16           # tile width of the tile engine (cells along x)
+TILE_RESERVE = 512    # workspace nodes the tile engine keeps for its step counters
+TILE_MIN_WS = 1024    # smallest workspace the tile engine accepts
+
+
+def tile_height(L: int) -> int:
+    """Tile height (cells along y) of the tile engine for L layers: 16 for
+    the layer counts 6, 8, 12, 16 and 20, else 10 (at most 5120 nodes)."""
+    return 16 if L in (6, 8, 12, 16, 20) else 10
+
+
+def tile_steps(box: np.ndarray, L: int) -> np.ndarray:
+    """Step budget of each net on the tile engine: its tile count along x
+    plus along y (about the tile distance from a corner to the opposite
+    one; a search still active after its wave's budget finishes in the
+    single-workgroup drain)."""
+    b = np.asarray(box).reshape(-1, 4).astype(np.int64)
+    ty = tile_height(L)
+    return ((b[:, 2] + TILE_X - 1) // TILE_X + (b[:, 3] + ty - 1) // ty).astype(np.int32)
 
 
 def _wave_offsets(roi_nodes: np.ndarray, nets: np.ndarray) -> tuple[np.ndarray, int]:
@@ -420,6 +442,7 @@ class DevicePathFinder:
         self._plan_off_h = torch.empty(max(1, n), dtype=torch.int64, pin_memory=True)
         self.stream.synchronize()
         self.distance = net_distance(self.spec, netlist.src, netlist.dst)
+        self._tile_steps = tile_steps(netlist.box, self.spec.L)
         self.schedule = self.cfg.schedule(self.spec.L)
         self.astar = self.cfg.astar_bounds(self.spec.base_cost_minima() if base is None else base)
         self.iteration = 0
@@ -486,7 +509,7 @@ class DevicePathFinder:
         not depend on delta; only time does."""
         c = self.cfg
         nets = st.routed + st.failed
-        if not c.adapt_delta or nets == 0 or st.rounds == 0 or math.isinf(c.delta):
+        if c.engine != "wave" or not c.adapt_delta or nets == 0 or st.rounds == 0 or math.isinf(c.delta):
             return
         if st.phases > 8 * nets and 4 * st.items < st.rounds * c.block and \
                 float(c.delta) * self.delta_mult * 4.0 <= c.delta_max:
@@ -535,6 +558,13 @@ class DevicePathFinder:
         waves = plan_waves(self.netlist.box, order, self.spec.X, self.spec.Y, self.nets.roi_nodes)
         return split_waves(waves, self.nets.roi_nodes, self.ws_cap())
 
+    def _wave_steps(self, flat_nets: np.ndarray, wave_ptr: np.ndarray) -> np.ndarray:
+        """Tile-engine step budget of each wave: its largest net's."""
+        if len(flat_nets) == 0:
+            return np.zeros(0, np.int32)
+        per = self._tile_steps[flat_nets]
+        return np.maximum.reduceat(per, wave_ptr[:-1]).astype(np.int32)
+
     def ws_cap(self) -> int:
         """Workspace nodes one launch may use (at least the largest ROI)."""
         r64 = (int(self.nets.roi_nodes.max(initial=1)) + 63) // 64 * 64
@@ -578,7 +608,7 @@ class DevicePathFinder:
         self._plan_nets_h.numpy()[:m] = flat_nets
         self._plan_off_h.numpy()[:m] = flat_off
         with torch.cuda.stream(stream):
-            self.ws.ensure(need)
+            self.ws.ensure(need + (TILE_RESERVE if self.cfg.engine == "tiles" else 0))
             d_nets = self._plan_nets[:max(m, 1)]
             d_off = self._plan_off[:max(m, 1)]
             if m:
@@ -590,15 +620,25 @@ class DevicePathFinder:
             rcfg = self.route_cfg()
             ev[1].record(stream)
             exec_graph = None
-            route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct), d_nets.data_ptr(),
-                          d_off.data_ptr(), len(waves), wave_ptr.ctypes.data, wave_max.ctypes.data,
-                          wave_ws.ctypes.data, C.byref(self.ws.struct), C.byref(rcfg))
+            if self.cfg.engine == "tiles":
+                entry = "orx_route_tiles"
+                wave_steps = self._wave_steps(flat_nets, wave_ptr)
+                route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
+                              d_nets.data_ptr(), d_off.data_ptr(), len(waves), wave_ptr.ctypes.data,
+                              wave_ws.ctypes.data, wave_steps.ctypes.data, C.byref(self.ws.struct), C.byref(rcfg))
+            elif self.cfg.engine == "wave":
+                entry = "orx_route_waves"
+                route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
+                              d_nets.data_ptr(), d_off.data_ptr(), len(waves), wave_ptr.ctypes.data,
+                              wave_max.ctypes.data, wave_ws.ctypes.data, C.byref(self.ws.struct), C.byref(rcfg))
+            else:
+                raise ValueError("engine must be 'tiles' or 'wave'")
             if graph and len(waves):
                 cs = C.c_void_p()
                 _lib.call("orx_graph_begin", C.byref(cs))
                 g = C.c_void_p()
                 try:
-                    _lib.call("orx_route_waves", *route_args, cs)
+                    _lib.call(entry, *route_args, cs)
                 except Exception:
                     # end the capture (which frees the stream) and drop the graph
                     if _lib.load().orx_graph_end(cs, C.byref(g)) == _lib.ORX_OK:
@@ -612,7 +652,7 @@ class DevicePathFinder:
                     _lib.load().orx_graph_destroy(exec_graph)
                     raise
             else:
-                _lib.call("orx_route_waves", *route_args, s)
+                _lib.call(entry, *route_args, s)
             ev[2].record(stream)
             _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
                       self._iter_out.data_ptr(), s)

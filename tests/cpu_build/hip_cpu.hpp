@@ -34,11 +34,16 @@
 #define __constant__
 #define __host__
 #define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(x)
 #define __shared__ static
 #include <map>
 
 struct alignas(16) float4 { float x, y, z, w; };
+struct alignas(8) float2 { float x, y; };
+inline float2 make_float2(float a, float b) { return float2{a, b}; }
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
 struct alignas(16) int4 { int x, y, z, w; };
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 struct alignas(16) ulonglong2 { unsigned long long x, y; };
@@ -211,10 +216,17 @@ T __shfl_xor(T v, int m) {
     return r;
 }
 inline uint64_t __ballot(bool p) { return hipcpu::ballot(p); }
+// direct global -> LDS load: lane l writes 16 bytes at lds_wave_base + 16 l
+inline void lds_load16(const void* g, void* lds_wave_base) {
+    memcpy(static_cast<unsigned char*>(lds_wave_base) + 16 * hipcpu::lane(), g, 16);
+}
+inline void lds_load_wait() {}
 #define __builtin_amdgcn_mbcnt_lo(m, a) hipcpu::mbcnt_lo((m), (a))
 #define __builtin_amdgcn_mbcnt_hi(m, a) hipcpu::mbcnt_hi((m), (a))
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
+#define __builtin_amdgcn_sched_barrier(n) ((void)0)
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
+inline int __ffs(unsigned v) { return __builtin_ffs((int)v); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((uint64_t)a * b) >> 32); }
@@ -283,6 +295,7 @@ hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, size_t) 
 inline hipError_t hipMallocAsync(void** p, size_t b, hipStream_t) { *p = malloc(b); return hipSuccess; }
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { free(p); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t b, int, hipStream_t) { memcpy(d, s, b); return 0; }
+inline hipError_t hipMemsetAsync(void* d, int v, size_t b, hipStream_t) { memset(d, v, b); return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 // streams run synchronously here, so events are always complete
 typedef void* hipEvent_t;

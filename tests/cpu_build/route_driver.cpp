@@ -89,7 +89,7 @@ int main(int argc, char** argv) {
     auto dep_ids = load<int32_t>("dep_ids", (size_t)std::max<int64_t>(1, M("n_deps")));
     auto ws_off = load<int64_t>("ws_off", n);
     auto done = load<int32_t>("done", n);
-    const int64_t wsn = M("ws_nodes");
+    const int64_t wsn = std::max<int64_t>(M("ws_nodes") + 512, 1024);
     std::vector<uint64_t> keys((size_t)wsn);
     std::vector<uint32_t> queues((size_t)(4 * wsn));
     std::vector<uint32_t> bitmaps((size_t)(3 * wsn / 32));
@@ -125,7 +125,7 @@ int main(int argc, char** argv) {
     // 0 = dependency-driven launch, 1 = one launch per net in sched order,
     // 2 = the wave sequence through orx_route_waves (sched is ordered by wave)
     const int64_t mode = M("mode");
-    if (mode == 2) {
+    if (mode == 2 || mode == 3) {
         auto wave = load<int32_t>("wave", n);
         std::vector<int32_t> ptr{0};
         std::vector<int64_t> wmax, wws, offs((size_t)n);
@@ -142,8 +142,27 @@ int main(int argc, char** argv) {
             mx = std::max(mx, R);
         }
         ptr.push_back(n); wmax.push_back(mx); wws.push_back(acc);
-        CHECK(orx_route_waves(&lat, &P, &nets, sched.data(), offs.data(), (int32_t)wmax.size(), ptr.data(),
-                              wmax.data(), wws.data(), &ws, &cfg, nullptr));
+        if (mode == 2) {
+            CHECK(orx_route_waves(&lat, &P, &nets, sched.data(), offs.data(), (int32_t)wmax.size(), ptr.data(),
+                                  wmax.data(), wws.data(), &ws, &cfg, nullptr));
+        } else {
+            // tile engine: step budget per wave = the largest tile count along
+            // x plus along y, capped by tile_steps_cap (< 0: no cap), so
+            // the single-workgroup drain runs the rest
+            const int64_t capk = g_meta.count("tile_steps_cap") ? M("tile_steps_cap") : -1;
+            const int32_t ty = (L == 6 || L == 8 || L == 12 || L == 16 || L == 20) ? 16 : 10;
+            std::vector<int32_t> steps(wmax.size(), 0);
+            for (size_t w = 0; w < wmax.size(); ++w) {
+                for (int32_t i = ptr[w]; i < ptr[w + 1]; ++i) {
+                    const int32_t k = sched[i];
+                    const int32_t bw = box[4 * (size_t)k + 2], bh = box[4 * (size_t)k + 3];
+                    steps[w] = std::max(steps[w], (bw + 15) / 16 + (bh + ty - 1) / ty);
+                }
+                if (capk >= 0) steps[w] = std::min<int32_t>(steps[w], (int32_t)capk);
+            }
+            CHECK(orx_route_tiles(&lat, &P, &nets, sched.data(), offs.data(), (int32_t)wmax.size(), ptr.data(),
+                                  wws.data(), steps.data(), &ws, &cfg, nullptr));
+        }
     } else if (mode == 0) {
         CHECK(orx_route_dag(&lat, &P, &nets, sched.data(), n, dep_ptr.data(), dep_ids.data(), done.data(),
                             (int32_t)M("epoch"), counter.data(), ws_off.data(), M("max_nodes"), &ws, &cfg,

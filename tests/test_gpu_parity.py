@@ -53,10 +53,11 @@ def _run_both(nl, cfg, iters, mode, device):
     return dev, orc
 
 
+@pytest.mark.parametrize("engine", ["tiles", "wave"])
 @pytest.mark.parametrize("mode", ["batched", "plain"])
-def test_plain_64_ten_iterations_bit_exact(device, mode):
+def test_plain_64_ten_iterations_bit_exact(device, mode, engine):
     nl = plain_64()
-    cfg = RouterConfig(iterations=10)
+    cfg = RouterConfig(iterations=10, engine=engine)
     dev, orc = _run_both(nl, cfg, 10, mode, device)
     assert dev.history[-1].overuse == orc.history[-1]["edge_over"] + orc.history[-1]["node_over"]
 
@@ -66,7 +67,7 @@ def test_plain_64_ten_iterations_bit_exact(device, mode):
 @pytest.mark.parametrize("delta", [1.0, 3.0, float("inf")])
 def test_uneven_small_bit_exact(device, lds_nodes, delta, astar):
     nl = uneven(96, 80, 8, 300, seed=11)
-    cfg = RouterConfig(iterations=4, delta=delta, lds_nodes=lds_nodes, astar=astar)
+    cfg = RouterConfig(iterations=4, delta=delta, lds_nodes=lds_nodes, astar=astar, engine="wave")
     _run_both(nl, cfg, 4, "batched", device)
 
 
@@ -74,27 +75,30 @@ def test_uneven_small_bit_exact(device, lds_nodes, delta, astar):
 def test_wide_bucket_large_block_bit_exact(device, lds_nodes):
     """The tuned launch shape (wide bucket, 1024 lanes) on a congested board."""
     nl = uneven(96, 80, 8, 300, seed=12)
-    cfg = RouterConfig(iterations=3, delta=16.0, block=1024, lds_nodes=lds_nodes)
+    cfg = RouterConfig(iterations=3, delta=16.0, block=1024, lds_nodes=lds_nodes, engine="wave")
     _run_both(nl, cfg, 3, "batched", device)
 
 
+@pytest.mark.parametrize("engine", ["tiles", "wave"])
 @pytest.mark.parametrize("block", [256, 512, 1024])
-def test_block_sizes_bit_exact(device, block):
+def test_block_sizes_bit_exact(device, block, engine):
     nl = uneven(64, 64, 6, 150, seed=5)
-    cfg = RouterConfig(iterations=3, block=block)
+    cfg = RouterConfig(iterations=3, block=block, engine=engine)
     _run_both(nl, cfg, 3, "batched", device)
 
 
-def test_largest_rois_bit_exact(device):
-    """The 16 largest nets of the 800x800x20 board (ROIs of 0.5-2.5 M nodes):
-    keys and bitmaps in the global workspace, the near queue spilling past
-    its LDS ring, the touched-node reset -- all against the oracle."""
+@pytest.mark.parametrize("engine", ["tiles", "wave"])
+def test_largest_rois_bit_exact(device, engine):
+    """The 16 largest nets of the 800x800x20 board (ROIs of 0.5-2.5 M nodes)
+    against the oracle: on the tile engine, hundreds of tiles per net over
+    tens of steps; on the per-net engine, keys and bitmaps in the global
+    workspace, the near queue spilling past its LDS ring."""
     from orthoroute_amd.netgen import uneven_800
     nl = uneven_800()
     big = np.argsort(-nl.roi_nodes(), kind="stable")[:16]
     sub = nl.select(np.sort(big), "largest16")
     assert sub.roi_nodes().min() > 400_000
-    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384)
+    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384, engine=engine)
     _run_both(sub, cfg, 2, "batched", device)
 
 

@@ -176,10 +176,11 @@ def test_native_planner_on_a_subset_order():
 
 
 def test_adaptive_delta_widens_only_after_many_thin_phases():
-    """Delta grows x4 after an iteration with more than 8 threshold phases
-    per net and rounds under a quarter of a workgroup (the congested case);
-    an iteration whose searches end in their first bucket leaves it, and it
-    stops at delta_max."""
+    """Per-net engine: delta grows x4 after an iteration with more than 8
+    threshold phases per net and rounds under a quarter of a workgroup (the
+    congested case); an iteration whose searches end in their first bucket
+    leaves it, and it stops at delta_max.  The tile engine has no buckets:
+    it never widens."""
     from types import SimpleNamespace
 
     from orthoroute_amd.router import DevicePathFinder, IterationStats, RouterConfig
@@ -189,7 +190,10 @@ def test_adaptive_delta_widens_only_after_many_thin_phases():
                               node_over=0, node_over_count=0, via_hops=0, barrels=0, items=items, rounds=rounds,
                               phases=phases)
 
-    job = SimpleNamespace(cfg=RouterConfig(delta=16.0, block=1024), delta_mult=1.0)
+    tiles = SimpleNamespace(cfg=RouterConfig(delta=16.0, block=1024, engine="tiles"), delta_mult=1.0)
+    DevicePathFinder._adapt_delta(tiles, stats(phases=2700, items=48_000, rounds=1000))
+    assert tiles.delta_mult == 1.0
+    job = SimpleNamespace(cfg=RouterConfig(delta=16.0, block=1024, engine="wave"), delta_mult=1.0)
     DevicePathFinder._adapt_delta(job, stats(phases=5, items=60_000, rounds=1000))      # A*: first bucket
     assert job.delta_mult == 1.0
     DevicePathFinder._adapt_delta(job, stats(phases=2700, items=48_000, rounds=1000))   # thin phases
