@@ -537,12 +537,34 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void route_wave_kernel(Lat g, Params P, NetsDev nets, const int32_t* wave_nets,
                                                            const int64_t* ws_off, orx_workspace ws, float delta,
                                                            int32_t lds_nodes, int32_t lds_region, int32_t flags,
-                                                           unsigned long long* stats) {
+                                                           unsigned long long* stats, int64_t ws_limit,
+                                                           int64_t max_nodes) {
     ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(Ctl, ctl);
     ORX_SHARED(TermTab, tt);
-    route_net<BLOCK>(g, P, nets, wave_nets[blockIdx.x], ws, ws_off[blockIdx.x], delta, lds_nodes, lds_region, flags,
-                     stats, reinterpret_cast<uint32_t*>(lds_mem), ctl, tt);
+    // The plan's net id and workspace slice must fit: a net id below n, the
+    // ROI no larger than the launch's largest (which sized the LDS plan and
+    // the prefilled range), its slice 64-aligned inside [0, ws_limit).  A
+    // slot that does not is refused (status ORX_E_ARG, counted as failed).
+    const int32_t k = wave_nets[blockIdx.x];
+    const int64_t off = ws_off[blockIdx.x];
+    bool ok = k >= 0 && k < nets.n;
+    if (ok) {
+        const int64_t R = (int64_t)nets.box[4 * (int64_t)k + 2] * nets.box[4 * (int64_t)k + 3] * g.L;
+        ok = R >= 1 && R <= max_nodes && off >= 0 && (off & 63) == 0 && off + R <= ws_limit;
+    }
+    if (!ok) {
+        if (threadIdx.x == 0) {
+            if (k >= 0 && k < nets.n) {
+                nets.status[k] = ORX_E_ARG;
+                nets.path_len[k] = 0;
+            }
+            if (stats) atomicAdd(stats + 6, 1ull);
+        }
+        return;
+    }
+    route_net<BLOCK>(g, P, nets, k, ws, off, delta, lds_nodes, lds_region, flags, stats,
+                     reinterpret_cast<uint32_t*>(lds_mem), ctl, tt);
 }
 
 // Persistent dependency-driven form: workgroups claim nets in schedule order
@@ -1249,6 +1271,7 @@ int launch_route_wave(const Lat& g, const Params& P, const NetsDev& nd, const in
     unsigned long long* st = (unsigned long long*)cfg->stats;
     int rc = ORX_OK;
     int32_t flags = cfg->flags & ~kPrefilled;
+    const int64_t limit = ws_extent > 0 ? std::min<int64_t>(ws_extent, ws->ws_nodes) : ws->ws_nodes;
     if (ws_extent > 0) {
         // whether a net keeps keys / bitmaps in the workspace grows with its
         // ROI, so the largest ROI of the wave decides
@@ -1269,7 +1292,7 @@ int launch_route_wave(const Lat& g, const Params& P, const NetsDev& nd, const in
         rc = set_lds<route_wave_kernel<BS>>(lds);                                                         \
         if (rc) return rc;                                                                                \
         hipLaunchKernelGGL(route_wave_kernel<BS>, dim3(K), dim3(BS), lds, s, g, P, nd, d_wave_nets,       \
-                           d_ws_off, *ws, cfg->delta, lp.lds_nodes, lp.region, flags, st);                \
+                           d_ws_off, *ws, cfg->delta, lp.lds_nodes, lp.region, flags, st, limit, max_nodes); \
     } while (0)
     switch (cfg->block) {
     case 256: ORX_LAUNCH_ROUTE(256); break;

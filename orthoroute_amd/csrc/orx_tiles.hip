@@ -181,7 +181,7 @@ __device__ __forceinline__ void queue_tile(const TileStep& S, uint32_t s, int64_
 struct TileShared {
     uint32_t changed[3];
     uint32_t act[4];   // halo of the left / right / lower / upper neighbour changed
-    uint32_t first;    // first visit of the tile in this search
+    uint32_t first;    // first visit of the tile in this search (its finite keys are unrelaxed seeds)
 };
 
 // One relaxation step of a scan: node key k receives the carry c (kept if
@@ -315,13 +315,13 @@ constexpr int32_t kPerLane = (16 * 16 * 20 + kTileBlock - 1) / kTileBlock;
 // The fixpoint phases of a visit.  Planar phase: the lanes of the rows of
 // the H layers first, then (from a multiple of 64, so every wave scans one
 // orientation) the lanes of the columns of the V layers.  A line is scanned
-// when its pm byte is set (a node of it changed in the via phase before)
-// or, in the first phase, when its halo carry improves its end node (or on
-// the tile's first visit: everything); the via phase likewise scans the
+// when its pm byte is set (a node of it changed in the via phase before, or
+// it holds a seed on the tile's first visit) or, in the first phase, when
+// its halo carry improves its end node; the via phase likewise scans the
 // columns in vm.  Every changed node marks its other line, its dirty byte,
 // and the neighbour whose halo it is.  Returns whether a key changed.
 template <int NT, int LC, bool MT>
-__device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, bool first, int32_t L, int32_t tw,
+__device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t L, int32_t tw,
                                            int32_t th, uint64_t* K, const float4* W4, const uint64_t* CM,
                                            const uint64_t* CP, const unsigned char* TG, unsigned char* DT,
                                            unsigned char* PM, unsigned char* VM, TileShared& sh) {
@@ -344,9 +344,7 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, bool fi
             const int32_t b0 = horiz ? (z * TY + l) * kPitch : z * ZS + l;
             const int32_t st = horiz ? 1 : kPitch;
             const uint64_t cmi = CM[li], cpi = CP[li];
-            bool need;
-            if (j == 0) need = first || cmi < K[b0] || cpi < K[b0 + (n - 1) * st];
-            else need = PM[li] != 0;
+            const bool need = PM[li] != 0 || (j == 0 && (cmi < K[b0] || cpi < K[b0 + (n - 1) * st]));
             if (!need) continue;
             PM[li] = 0;
             uint32_t tg = 0;
@@ -367,7 +365,7 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, bool fi
         }
     } else if ((int32_t)tid < kTX * TY) {
         const int32_t lx = (int32_t)tid & (kTX - 1), ly = (int32_t)tid >> 4;
-        if (lx < tw && ly < th && ((j == 1 && first) || VM[tid])) {
+        if (lx < tw && ly < th && VM[tid]) {
             VM[tid] = 0;
             const int32_t b0 = ly * kPitch + lx;
             uint32_t tg = 0;
@@ -429,7 +427,7 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
     uint32_t seen_w = 0;
     if (tid == 0) {
         __hip_atomic_fetch_and(S.marks[pin] + G.off / 32 + (t >> 5), ~tbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        seen_w = gload_u32(seen);
+        seen_w = __hip_atomic_fetch_or(seen, tbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh.changed[0] = sh.changed[1] = sh.changed[2] = 0;
         sh.act[0] = sh.act[1] = sh.act[2] = sh.act[3] = 0;
     }
@@ -508,7 +506,22 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
     if (tid == 0) sh.first = (seen_w & tbit) == 0;
     lds_load_wait();
     __syncthreads();
-    const bool first = sh.first != 0;
+    // The tile's keys are a fixpoint for the halo of its last visit (no one
+    // else writes them), so only lines whose carry improves their end node
+    // need a scan -- except on its first visit, when finite keys are seeds
+    // not yet relaxed: their line and column are marked.
+    if (sh.first) {
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u) {
+            const int32_t i = (int32_t)tid + u * NT;
+            if (i < span && lk[u] != ORX_KEY_INF) {
+                const int32_t lx = i & (kTX - 1), q = i >> 4, z = q / TY, ly = q - z * TY;
+                PM[z * kLines + (((S.hmask >> z) & 1u) ? ly : lx)] = 1;
+                VM[ly * kTX + lx] = 1;
+            }
+        }
+        __syncthreads();
+    }
     // 2. targets of a multi-terminal net are sinks: reached, never expanded
     // (UP:2611-2618).  Mark those in the tile; a target on the halo gives
     // its line no carry.
@@ -539,8 +552,8 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
     // visit, and only the carries are new).
     int32_t j = 0;
     for (;; ++j) {
-        const bool chg = multi ? tile_phase<NT, LC, true>(S, j, first, L, tw, th, K, W4, CM, CP, TG, DT, PM, VM, sh)
-                               : tile_phase<NT, LC, false>(S, j, first, L, tw, th, K, W4, CM, CP, TG, DT, PM, VM, sh);
+        const bool chg = multi ? tile_phase<NT, LC, true>(S, j, L, tw, th, K, W4, CM, CP, TG, DT, PM, VM, sh)
+                               : tile_phase<NT, LC, false>(S, j, L, tw, th, K, W4, CM, CP, TG, DT, PM, VM, sh);
         if (chg) sh.changed[j % 3] = 1u;
         __syncthreads();
         const bool any = sh.changed[j % 3] != 0;
@@ -559,7 +572,6 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
         if (nx >= 0 && nx < G.ntx && ny >= 0 && ny < G.nty)
             queue_tile(S, s, G.off, (uint32_t)(ny * G.ntx + nx), pout, cnt_out);
     }
-    if (tid == 0 && first) atomicOr(seen, tbit);
     // the write-back recomputes its indices from an opaque copy of the lane
     // id, so the compiler does not keep the load loop's indices alive
     // through the fixpoint
@@ -611,11 +623,15 @@ __global__ __launch_bounds__(BLOCK) void tile_prep_kernel(TileArgs A, int32_t ri
     const TileStep& S = A.S;
     const uint32_t s = blockIdx.x, tid = threadIdx.x;
     const Lat& g = A.g;
+    const NetsDev& nets = A.nets;
+    const bool wave = S.slot_net != nullptr;
+    if (wave && (S.slot_net[s] < 0 || S.slot_net[s] >= nets.n)) {   // a broken plan: refused
+        if (tid == 0) atomicOr(S.info + 3, 16ull);
+        return;
+    }
     const SlotGeo G = slot_geo(S, s);
     const Box& B = G.B;
-    const NetsDev& nets = A.nets;
     const int32_t k = G.k;
-    const bool wave = S.slot_net != nullptr;
     if (rip && wave) {
         int32_t* path = nets.path_arena + nets.path_off[k];
         const int32_t old_len = nets.path_len[k];
@@ -753,6 +769,10 @@ __global__ __launch_bounds__(BLOCK) void tile_post_kernel(TileArgs A) {
     const Lat& g = A.g;
     const Params& P = A.P;
     const NetsDev& nets = A.nets;
+    if (S.slot_net[s] < 0 || S.slot_net[s] >= nets.n) {   // refused by the prep
+        if (tid == 0 && A.stats) atomicAdd(A.stats + 6, 1ull);
+        return;
+    }
     const SlotGeo G = slot_geo(S, s);
     const Box& B = G.B;
     const int32_t k = G.k;

@@ -15,9 +15,10 @@
 All return global node ids (source first) or ``None`` when no path exists,
 like the reference.  ``try_attach_hip`` grafts the accelerator onto a
 SimpleDijkstra the way try_attach_metal does (metal_dijkstra.py:44-89): the
-wrapped methods call the HIP solver first and hand a ``None`` result to the
-solver's own original method, so a net the caller could route is never
-dropped.  Inside this package there is no CPU search: when liborx_hip.so or
+wrapped methods call the HIP solver first and hand a ``None`` result, or
+any exception it raises, to the solver's own original method, so a net the
+caller could route is never dropped.  Every query's shapes and node ids
+are checked on the host first (``check_search_args``).  Inside this package there is no CPU search: when liborx_hip.so or
 the GPU is missing the constructor raises.
 """
 from __future__ import annotations
@@ -51,6 +52,44 @@ def _in_roi(global_to_roi, nodes) -> np.ndarray:
         t = torch.from_numpy(idx).to(global_to_roi.device)
         return (global_to_roi[t] >= 0).cpu().numpy()
     return _host(global_to_roi)[idx] >= 0
+
+
+def _n(a) -> int:
+    return int(a.numel()) if isinstance(a, torch.Tensor) else len(np.asarray(_host(a)).reshape(-1))
+
+
+def check_search_args(N: int, E: int, R: int, costs, roi_nodes, g2r, pen, allowed, seeds, seed_cost, targets,
+                      tg_cost) -> None:
+    """Shapes and ranges of one seam query, checked on the host before any
+    device work (the kernel would index seed_cost[i], tg_cost[i], pen[r],
+    g2r[gid] and allowed[gid >> 5] with them).  Raises ValueError like the
+    reference does for mismatched terminal costs and node penalties
+    (cuda_dijkstra.py:5566-5570, 5594-5598)."""
+    if costs is None or _n(costs) != E:
+        raise ValueError(f"costs has {0 if costs is None else _n(costs)} entries, graph has {E} arcs")
+    if (roi_nodes is None) != (g2r is None):
+        raise ValueError("roi_nodes and global_to_roi go together")
+    if roi_nodes is not None:
+        if _n(roi_nodes) != R:
+            raise ValueError(f"roi_nodes has {_n(roi_nodes)} entries, expected {R}")
+        if _n(g2r) != N:
+            raise ValueError(f"global_to_roi has {_n(g2r)} entries, graph has {N} nodes")
+        rn = np.asarray(_host(roi_nodes)).reshape(-1)
+        if rn.size and (rn.min() < 0 or rn.max() >= N):
+            raise ValueError("roi_nodes holds a node id outside the graph")
+    elif R != N:
+        raise ValueError("a whole-graph query has R = N")
+    if pen is not None and _n(pen) != R:
+        raise ValueError(f"node_penalty has {_n(pen)} entries, expected {R} "
+                         f"({'ROI-local' if roi_nodes is not None else 'global'})")
+    if allowed is not None and _n(allowed) < (N + 31) // 32:
+        raise ValueError(f"allowed_bitmap has {_n(allowed)} words, needs {(N + 31) // 32}")
+    for name, ids, cost in (("seed", seeds, seed_cost), ("target", targets, tg_cost)):
+        ids = np.asarray(ids, np.int64).reshape(-1)
+        if ids.size and (ids.min() < 0 or ids.max() >= N):
+            raise ValueError(f"a {name} node id is outside the graph")
+        if cost is not None and _n(cost) != ids.size:
+            raise ValueError(f"{name} costs have {_n(cost)} entries for {ids.size} {name}s")
 
 
 class HipDijkstra:
@@ -108,10 +147,10 @@ class HipDijkstra:
     # -- core ----------------------------------------------------------------
     def _search(self, costs, R, roi_nodes, g2r, pen, allowed, seeds, seed_cost, targets, tg_cost,
                 seed_pen: bool) -> Optional[List[int]]:
+        check_search_args(self.N, self.E, R, costs, roi_nodes, g2r, pen, allowed, seeds, seed_cost, targets,
+                          tg_cost)
         self._ensure(R)
         d_costs = self._dev(costs, torch.float32)
-        if d_costs.numel() != self.E:
-            raise ValueError(f"costs has {d_costs.numel()} entries, graph has {self.E} arcs")
         d_seeds = self._dev(np.asarray(seeds, np.int32), torch.int32)
         d_sc = self._dev(seed_cost, torch.float32) if seed_cost is not None else None
         d_tg = self._dev(np.asarray(targets, np.int32), torch.int32)
@@ -190,40 +229,66 @@ class HipDijkstra:
                             seed_pen=True)
 
 
-def try_attach_hip(solver, graph, lattice=None, *, device="cuda") -> bool:
-    """Graft the HIP solver onto a SimpleDijkstra-like object (the
-    ORTHO_BACKEND=hip counterpart of try_attach_metal, metal_dijkstra.py:44-89).
-
-    Sets ``solver.gpu_solver`` (unified_pathfinder.py:3099-3120) and wraps
-    ``find_path_roi`` / ``find_path_multisource_multisink`` so a ``None``
-    from the HIP solver falls through to the solver's own original method.
-    Returns False (solver untouched) when the library or GPU is unavailable.
-    """
-    try:
-        hip = HipDijkstra(graph, lattice, device=device)
-    except (_lib.HipUnavailable, RuntimeError, OSError) as e:
-        log.info("[HIP] accelerator not attached: %s", e)
-        return False
+def graft(solver, hip) -> None:
+    """Wrap ``solver.find_path_roi`` / ``find_path_multisource_multisink`` so
+    the accelerator ``hip`` answers first, and set ``solver.gpu_solver``
+    (unified_pathfinder.py:3099-3120).  A ``None`` from the accelerator, or
+    any exception it raises (logged), hands the query to the solver's own
+    original method, so a net the CPU could route is never dropped
+    (metal_dijkstra.py:374-387 catches the same way).  The original method
+    runs with ``gpu_solver`` unset, so it does not search on the GPU a
+    second time (SimpleDijkstra.find_path_roi tries gpu_solver first for
+    large ROIs, UP:2465-2480)."""
     orig_roi = solver.find_path_roi
     orig_multi = getattr(solver, "find_path_multisource_multisink", None)
 
+    def on_cpu(fn, *args, **kw):
+        saved = getattr(solver, "gpu_solver", None)
+        solver.gpu_solver = None
+        try:
+            return fn(*args, **kw)
+        finally:
+            solver.gpu_solver = saved
+
     def roi(src, dst, costs, roi_nodes, global_to_roi, node_penalty=None):
-        path = hip.find_path_roi_gpu(src, dst, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+        try:
+            path = hip.find_path_roi_gpu(src, dst, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+        except Exception as e:   # noqa: BLE001 -- any accelerator failure goes to the CPU path
+            log.warning("[HIP] find_path_roi failed (%s); CPU fallback", e)
+            path = None
         if path is not None:
             return path
-        return orig_roi(src, dst, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+        return on_cpu(orig_roi, src, dst, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
 
     def multi(src_seeds, dst_targets, costs, roi_nodes, global_to_roi, node_penalty=None):
-        res = hip.find_path_multisource_multisink(src_seeds, dst_targets, costs, roi_nodes, global_to_roi,
-                                                  node_penalty=node_penalty)
-        if res is not None or orig_multi is None:
+        try:
+            res = hip.find_path_multisource_multisink(src_seeds, dst_targets, costs, roi_nodes, global_to_roi,
+                                                      node_penalty=node_penalty)
+        except Exception as e:   # noqa: BLE001
+            log.warning("[HIP] find_path_multisource_multisink failed (%s); CPU fallback", e)
+            res = None
+        if res is not None:
             return res
-        return orig_multi(src_seeds, dst_targets, costs, roi_nodes, global_to_roi, node_penalty=node_penalty)
+        return on_cpu(orig_multi, src_seeds, dst_targets, costs, roi_nodes, global_to_roi,
+                      node_penalty=node_penalty)
 
     solver.find_path_roi = roi
     if orig_multi is not None:
         solver.find_path_multisource_multisink = multi
     solver.gpu_solver = hip
     solver.hip_solver = hip
+
+
+def try_attach_hip(solver, graph, lattice=None, *, device="cuda") -> bool:
+    """Graft the HIP solver onto a SimpleDijkstra-like object (the
+    ORTHO_BACKEND=hip counterpart of try_attach_metal, metal_dijkstra.py:44-89).
+    Returns False (solver untouched) when the library or GPU is unavailable;
+    see ``graft`` for the wrapping."""
+    try:
+        hip = HipDijkstra(graph, lattice, device=device)
+    except (_lib.HipUnavailable, RuntimeError, OSError) as e:
+        log.info("[HIP] accelerator not attached: %s", e)
+        return False
+    graft(solver, hip)
     log.info("[HIP] gfx950 accelerator attached")
     return True

@@ -22,7 +22,13 @@ def build_driver() -> Path:
 
 
 class CpuBuildPathFinder:
-    def __init__(self, netlist, cfg, workdir: Path, mode: int = 0, max_groups: int = 0, base=None):
+    def __init__(self, netlist, cfg, workdir: Path, mode: int = 0, max_groups: int = 0, base=None,
+                 extra_meta: dict | None = None):
+        """mode 0: dependency-driven launch; 1: one launch per net; 2: the
+        wave sequence of orx_route_waves; 3: the same waves on the tile
+        engine (orx_route_tiles).  extra_meta: driver options
+        (tile_steps_cap, bad_slot)."""
+        self.extra_meta = dict(extra_meta or {})
         self.nl = netlist
         self.cfg = cfg
         self.spec = spec = netlist.spec
@@ -99,6 +105,7 @@ class CpuBuildPathFinder:
         T = self.nl.terms
         if T is not None:
             meta.update({"terms": 1, "n_seed": len(T.seed_node), "n_tgt": len(T.tgt_node)})
+        meta.update(self.extra_meta)
         with open(self.dir / "meta.txt", "w") as f:
             for k, v in meta.items():
                 f.write(f"{k} {float(v)!r}\n")

@@ -142,6 +142,9 @@ int main(int argc, char** argv) {
             mx = std::max(mx, R);
         }
         ptr.push_back(n); wmax.push_back(mx); wws.push_back(acc);
+        // a broken plan for the refusal tests: one slot's workspace offset
+        // off its 64-node alignment
+        if (g_meta.count("bad_slot")) offs[(size_t)M("bad_slot")] += 1;
         if (mode == 2) {
             CHECK(orx_route_waves(&lat, &P, &nets, sched.data(), offs.data(), (int32_t)wmax.size(), ptr.data(),
                                   wmax.data(), wws.data(), &ws, &cfg, nullptr));

@@ -254,3 +254,71 @@ def test_roi_bfs_on_host_threads(driver, tmp_path):
         assert out[1] == info["depth"] and out[3] == info["seen"] == out[5], i
         assert (out[2] if out[2] >= 0 else None) == info["meeting_depth"], i
         assert out[4] == 0 and clean.tolist() == [0, 0], i
+
+
+@pytest.mark.parametrize("X,Y,L,n,seed,conc,cap", [
+    (40, 34, 6, 16, 4, 1, -1),     # class 6, every step launch
+    (40, 34, 6, 16, 4, 3, 1),      # concurrent workgroups; one step, the rest in the drain
+    (30, 26, 5, 12, 7, 2, 0),      # generic class (5 layers): everything in the drain
+    (34, 30, 20, 6, 2, 2, -1),     # class 20 (the 800 x 800 x 20 tiles)
+])
+def test_tile_engine_on_host_threads(driver, tmp_path, monkeypatch, X, Y, L, n, seed, conc, cap):
+    """The tile engine (orx_route_tiles): boxes of several tiles, step
+    budgets from ample to none (the single-workgroup drain finishes the
+    search), workgroups run one at a time or concurrently -- paths, fp32
+    costs, occupancy, history and settled / relaxed counts equal the
+    oracle's, with no out-of-bounds access."""
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", str(conc))
+    nl = uneven(X, Y, L, n, seed=seed)
+    cfg = RouterConfig(iterations=2)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3, extra_meta={"tile_steps_cap": cap})
+    o = OraclePathFinder(nl, cfg)
+    for _ in range(2):
+        d = c.run_iteration(timeout=600)
+        od = o.run_iteration()
+        for k in ("routed", "failed", "edge_over", "node_over", "via_hops", "barrels", "settled", "relaxed",
+                  "path_hops"):
+            assert d[k] == od[k], k
+        for a, b in zip(c.paths(), o.paths()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(c.path_cost.view(np.uint32), o.path_cost.view(np.uint32))
+        assert np.array_equal(c.occ, o.occ)
+        assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
+
+
+def test_tile_engine_portal_nets_on_host_threads(driver, tmp_path, monkeypatch):
+    """Multi-terminal nets on the tile engine: targets are sinks (never
+    expanded, also across a tile boundary), the best target by total."""
+    from orthoroute_amd.netgen import with_portals
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "2")
+    nl = with_portals(uneven(36, 30, 6, 10, seed=4), candidates=2, fraction=0.7, seed=1)
+    cfg = RouterConfig(iterations=2)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3)
+    o = OraclePathFinder(nl, cfg)
+    for _ in range(2):
+        d = c.run_iteration(timeout=600)
+        od = o.run_iteration()
+        for k in ("routed", "failed", "edge_over", "node_over", "settled", "relaxed", "path_hops"):
+            assert d[k] == od[k], k
+        for a, b in zip(c.paths(), o.paths()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(c.path_cost.view(np.uint32), o.path_cost.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_broken_plan_is_refused(driver, tmp_path, mode):
+    """A plan whose workspace offset of one slot is off its 64-node
+    alignment: that net is refused (ORX_E_ARG, no path, nothing committed)
+    on both engines, and nothing reads or writes outside the buffers."""
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    nl = uneven(24, 20, 3, 6, seed=9)
+    cfg = RouterConfig(iterations=1)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=mode, extra_meta={"bad_slot": 1})
+    d = c.run_iteration(timeout=300)
+    st = d["status"]
+    assert (st == -1).sum() == 1
+    k = int(np.flatnonzero(st == -1)[0])
+    assert c.path_len[k] == 0
+    assert d["failed"] >= 1
