@@ -294,6 +294,21 @@ int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int3
                  int64_t src, int64_t dst, float* d_dist, int32_t* d_parent,
                  const orx_workspace* ws, const orx_route_cfg* cfg, orx_stream_t stream);
 
+/* orx_sssp_box without any allocation (no counterpart in the reference):
+ * for callers that cannot call hipMalloc -- code under hipGraph capture,
+ * tight serving loops, harnesses that forbid allocation.  The same query,
+ * arrays and status as orx_sssp_box on its one-workgroup path, with the
+ * search scratch (keys, queues, bitmaps, status word) in a static device
+ * array of the library, one per GPU, held until the process ends.  ROIs up
+ * to ORX_NOALLOC_MAX_NODES nodes; larger ones, and ORX_F_GRID /
+ * ORX_F_TILES, are refused with ORX_E_ARG.  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (other streams, other
+ * threads) race.  Synchronous, like orx_sssp_box. */
+#define ORX_NOALLOC_MAX_NODES 65536
+int orx_sssp_box_noalloc(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int32_t h, int64_t src,
+                         int64_t dst, float* d_dist, int32_t* d_parent, const orx_route_cfg* cfg,
+                         orx_stream_t stream);
+
 /* Generic CSR query behind the gpu_solver seam:
  *   find_path_roi_gpu(src, dst, costs, roi_nodes, global_to_roi, node_penalty)
  *     (CD:4951-5040; UP:2465-2480), and

@@ -24,6 +24,7 @@ ORX_F_FULL = 2
 ORX_F_SEED_PEN = 4
 ORX_F_GRID = 8
 ORX_F_TILES = 16
+ORX_NOALLOC_MAX_NODES = 65536
 
 STATUS_NAMES = {
     ORX_OK: "ok",
@@ -116,6 +117,8 @@ SIGNATURES = [
                                   C.c_int32, P]),
     ("orx_sssp_box", C.c_int, [C.POINTER(OrxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
                                C.c_int64, P, P, C.POINTER(OrxWorkspace), C.POINTER(OrxRouteCfg), P]),
+    ("orx_sssp_box_noalloc", C.c_int, [C.POINTER(OrxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int64, C.c_int64, P, P, C.POINTER(OrxRouteCfg), P]),
     ("orx_sssp_csr", C.c_int, [C.POINTER(OrxCsr), P, C.c_int64, P, P, P, P, C.c_int32, P, P, C.c_int32, P, P,
                                P, P, C.c_int64, P, C.POINTER(OrxRouteCfg), P]),
     ("orx_backtrace_csr", C.c_int, [C.POINTER(OrxCsr), P, P, P, P, C.c_int32, P, P]),

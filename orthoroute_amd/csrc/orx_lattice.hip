@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include "../../include/orthoroute_hip.h"
 
 using namespace orx;
@@ -1307,6 +1308,41 @@ int launch_route_wave(const Lat& g, const Params& P, const NetsDev& nd, const in
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Search scratch of orx_sssp_box_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends, so the entry point allocates
+// nothing.  Layout for up to ORX_NOALLOC_MAX_NODES ROI nodes: keys (u64),
+// four queues (u32), three bitmaps, the status word.  Host code never uses
+// the array's name as an address: each device's address comes from
+// hipGetSymbolAddress, once, with that device current.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocNodes = ORX_NOALLOC_MAX_NODES;
+constexpr size_t kScratchKeys = 8 * kNoAllocNodes;
+constexpr size_t kScratchQueues = 16 * kNoAllocNodes;
+constexpr size_t kScratchBitmaps = 3 * kNoAllocNodes / 8;
+constexpr size_t kScratchBytes = kScratchKeys + kScratchQueues + kScratchBitmaps + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// This device's copy of the scratch array (looked up on first use).
+int scratch_for_current_device(unsigned char** out) {
+    int dev = 0;
+    ORX_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return orx_host::fail(ORX_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        ORX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return orx_host::fail(ORX_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    return ORX_OK;
+}
+
 namespace orx_host {
 
 int prepare_lattice_kernels() {
@@ -1320,6 +1356,8 @@ int prepare_lattice_kernels() {
     if (!rc) rc = set_lds<sssp_box_kernel<256>>(kMaxDynLds);
     if (!rc) rc = set_lds<sssp_box_kernel<512>>(kMaxDynLds);
     if (!rc) rc = set_lds<sssp_box_kernel<1024>>(kMaxDynLds);
+    unsigned char* sp = nullptr;
+    if (!rc) rc = scratch_for_current_device(&sp);   // the allocation-free entry's scratch, this device
     return rc;
 }
 
@@ -1536,6 +1574,66 @@ int orx_sssp_box(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int3
     }
     (void)hipFreeAsync(d_status, s);
     return rc;
+}
+
+int orx_sssp_box_noalloc(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, int32_t h, int64_t src,
+                         int64_t dst, float* d_dist, int32_t* d_parent, const orx_route_cfg* cfg,
+                         orx_stream_t stream) {
+    if (!lat || !lat->wcost) return orx_host::fail(ORX_E_ARG, "lattice / wcost is NULL");
+    if (!cfg) return orx_host::fail(ORX_E_ARG, "NULL argument");
+    if (x0 < 0 || y0 < 0 || w < 1 || h < 1 || x0 + w > lat->X || y0 + h > lat->Y)
+        return orx_host::fail(ORX_E_ARG, "box outside the lattice");
+    const int64_t R = (int64_t)w * h * lat->L;
+    if (R > kNoAllocNodes)
+        return orx_host::fail(ORX_E_ARG, "ROI exceeds the allocation-free scratch (ORX_NOALLOC_MAX_NODES nodes)");
+    if (cfg->flags & (ORX_F_GRID | ORX_F_TILES))
+        return orx_host::fail(ORX_E_ARG, "the allocation-free query runs in one workgroup (no grid / tile flags)");
+    const int64_t XY = (int64_t)lat->X * lat->Y;
+    auto to_roi = [&](int64_t gid) -> int64_t {
+        if (gid < 0 || gid >= XY * lat->L) return -1;
+        const int64_t z = gid / XY, rem = gid - z * XY, y = rem / lat->X, x = rem - y * lat->X;
+        if (x < x0 || x >= x0 + w || y < y0 || y >= y0 + h) return -1;
+        return (z * h + (y - y0)) * w + (x - x0);
+    };
+    if (to_roi(src) < 0 || (dst >= 0 && to_roi(dst) < 0)) return orx_host::fail(ORX_E_ARG, "src / dst outside the box");
+    unsigned char* sp = nullptr;
+    int rc = scratch_for_current_device(&sp);
+    if (rc) return rc;
+    // the workspace slice of the scratch, sized to this ROI's node count
+    // rounded to 64 (every array of the one-workgroup search is cleared by
+    // the kernel itself; the status word is written by it)
+    orx_workspace ws;
+    ws.ws_nodes = kNoAllocNodes;
+    ws.keys = reinterpret_cast<uint64_t*>(sp);
+    ws.queues = reinterpret_cast<uint32_t*>(sp + kScratchKeys);
+    ws.bitmaps = reinterpret_cast<uint32_t*>(sp + kScratchKeys + kScratchQueues);
+    int32_t* d_status = reinterpret_cast<int32_t*>(sp + kScratchKeys + kScratchQueues + kScratchBitmaps);
+    const Lat g = make_lat(lat);
+    const hipStream_t s = (hipStream_t)stream;
+    const LdsPlan lp = lds_plan(R, cfg->lds_nodes);
+    const size_t lds = lp.bytes;
+#define ORX_LAUNCH_BOX(BS)                                                                                  \
+    do {                                                                                                    \
+        rc = set_lds<sssp_box_kernel<BS>>(lds);                                                             \
+        if (rc) break;                                                                                      \
+        hipLaunchKernelGGL(sssp_box_kernel<BS>, dim3(1), dim3(BS), lds, s, g, ws, x0, y0, w, h, src, dst,   \
+                           cfg->delta, lp.lds_nodes, lp.region, cfg->flags, d_dist, d_parent,               \
+                           (unsigned long long*)cfg->stats, d_status);                                      \
+    } while (0)
+    switch (cfg->block) {
+    case 256: ORX_LAUNCH_BOX(256); break;
+    case 512: ORX_LAUNCH_BOX(512); break;
+    case 1024: ORX_LAUNCH_BOX(1024); break;
+    default: rc = orx_host::fail(ORX_E_ARG, "block must be 256, 512 or 1024");
+    }
+#undef ORX_LAUNCH_BOX
+    if (rc) return rc;
+    int32_t h_status = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h_status, d_status, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return orx_host::check_hip(e, "orx_sssp_box_noalloc");
+    return h_status;
 }
 
 }  // extern "C"

@@ -32,6 +32,7 @@
 #define __global__
 #define __device__
 #define __constant__
+#define __align__(n) alignas(n)
 #define __host__
 #define __forceinline__ inline
 #define __noinline__ __attribute__((noinline))
@@ -290,6 +291,8 @@ inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) { strcpy(p->gcnArchName, "gfx950:cpu"); return 0; }
 inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = 4; return hipSuccess; }
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess; }
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 template <typename K>
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, size_t) { *n = 1; return hipSuccess; }
 inline hipError_t hipMallocAsync(void** p, size_t b, hipStream_t) { *p = malloc(b); return hipSuccess; }
