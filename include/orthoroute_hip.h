@@ -278,7 +278,8 @@ int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_c
  * arrays and re-price the touched arcs (EdgeAccountant.commit_path /
  * clear_path + refresh_live_present_costs, UP:1074-1135, 1403-1422;
  * _mark/_clear_path_node_use UP:4635-4659; via columns UP:4470-4546).
- * Paths must have pairwise distinct arcs across the batch. */
+ * Paths of one batch may share arcs and nodes (the updates are atomic and
+ * the re-pricing reads the final counts). */
 int orx_apply_paths(const orx_lattice* lat, const orx_params* p, const int32_t* d_paths,
                     const int64_t* d_off, const int32_t* d_len, int32_t K, int32_t sign,
                     orx_stream_t stream);
@@ -365,6 +366,78 @@ int orx_roi_release(const int32_t* d_roi, const int64_t* d_out, int64_t cap, int
 int orx_backtrace_csr(const orx_csr* g, const uint64_t* d_key, const int32_t* d_g2r,
                       const int32_t* d_target, int32_t* d_path, int32_t cap, int32_t* d_len,
                       orx_stream_t stream);
+
+/* ---- Per-arc accountant on the CSR arc order (EdgeAccountant, UP:1039-1422).
+ * Device arrays of E entries indexed by CSR arc id.  The arithmetic is the
+ * reference's float32 NumPy evaluation, operation for operation; coefficients
+ * are passed as the float32 roundings of the reference's Python floats.
+ * Every call is asynchronous on `stream`; outputs marked d_ are device memory
+ * the caller clears when they accumulate. */
+typedef struct orx_acct {
+    int64_t E;
+    float* present;      /* committed uses per arc (EdgeAccountant.present)      */
+    float* present_ema;  /* smoothed present                                    */
+    float* history;
+    float* capacity;
+    float* total_cost;   /* may be NULL until orx_acct_costs is used            */
+    float* live_static;  /* static part under live re-pricing (NULL if unused)  */
+    int32_t* canon;      /* canonical use count per arc (EdgeAccountant.canonical) */
+} orx_acct;
+
+/* commit_path (sign=+1, UP:1099-1120) / clear_path (sign=-1, UP:1122-1135)
+ * of the n arc ids d_idx (a batch of paths may be concatenated: the result
+ * equals committing / clearing them one by one).  floor_present applies the
+ * reference's whole-array floor np.maximum(present, 0) after a clear.  Ids
+ * outside [0, E) are skipped and counted into *d_bad (may be NULL). */
+int orx_acct_apply(const orx_acct* a, const int64_t* d_idx, int64_t n, int32_t sign, int32_t floor_present,
+                   unsigned long long* d_bad, orx_stream_t stream);
+/* compute_overuse's arc term (UP:1137-1167): *d_sum += sum of
+ * max(0, present - capacity) in float64, *d_count += arcs with nonzero
+ * overuse; d_mask (NULL = all) restricts both to the canonical arcs
+ * (_canonical_edge_resource_mask). */
+int orx_acct_overuse(const orx_acct* a, const uint8_t* d_mask, double* d_sum, unsigned long long* d_count,
+                     orx_stream_t stream);
+/* *d_out += float64 sum of: which=0 history; 1 max(0, present_ema - capacity)
+ * (cost_balance_ratio, UP:1054-1067); 2 |present - canon|
+ * (verify_present_matches_canonical, UP:1192-1224). */
+int orx_acct_sum(const orx_acct* a, int32_t which, double* d_out, orx_stream_t stream);
+/* update_present_ema (UP:1291-1299); one_minus_beta = float32(1 - beta). */
+int orx_acct_present_ema(const orx_acct* a, float beta, float one_minus_beta, orx_stream_t stream);
+/* update_history (UP:1226-1289); d_base NULL = no increment cap. */
+int orx_acct_history(const orx_acct* a, float gain, const float* d_base, float cap_mult, float decay,
+                     int32_t use_raw_present, orx_stream_t stream);
+/* update_costs (UP:1301-1384) into total_cost.  Layer bias applies when
+ * d_edge_layer, d_layer_bias[n_layers] and d_edge_kind are all given; then
+ * d_scale_out receives the per-arc present-cost scale pres_fac * bias.
+ * d_blocked (NULL = none) marks hard-blocked arcs (+inf, the _blocked_edges
+ * mask); jitter != 0 adds (arc mod 9973) * 1e-6. */
+int orx_acct_costs(const orx_acct* a, const float* d_base, float pres_fac, float hist_w, float via_mult,
+                   float base_w, const int32_t* d_edge_layer, const float* d_layer_bias, int32_t n_layers,
+                   const int32_t* d_edge_kind, const uint8_t* d_blocked, int32_t jitter, float* d_scale_out,
+                   orx_stream_t stream);
+/* begin_live_present_costs (UP:1386-1401) / refresh_live_present_costs
+ * (UP:1403-1422) with the present-cost scale `scale`, or per arc d_scale. */
+int orx_acct_begin_live(const orx_acct* a, float scale, const float* d_scale, orx_stream_t stream);
+int orx_acct_refresh_live(const orx_acct* a, const int64_t* d_idx, int64_t n, float scale, const float* d_scale,
+                          unsigned long long* d_bad, orx_stream_t stream);
+/* refresh_from_canonical (UP:1073-1097): present = canon. */
+int orx_acct_from_canonical(const orx_acct* a, orx_stream_t stream);
+
+/* Via capacity kernels on a CSR cost array (ViaKernelManager,
+ * pathfinder/via_kernels.py:186-325; arithmetic of the CPU forms
+ * via_kernels.py:567-651).  Via arc i: CSR id d_edge[i], column
+ * d_xy[2i], d_xy[2i+1] (column index x * Ny + y), segments z - 1 for z in
+ * [d_z_lo[i], d_z_hi[i]).  d_count[0] += arcs blocked / penalised,
+ * d_count[1] += entries with a column or arc id out of range (skipped).
+ * Segment arrays may be NULL (no segment term). */
+int orx_via_hard_block(const int32_t* d_edge, const int32_t* d_xy, const int32_t* d_z_lo, const int32_t* d_z_hi,
+                       int32_t n, int32_t Nx, int32_t Ny, int32_t segZ, const int16_t* d_col_use,
+                       const int16_t* d_col_cap, const int8_t* d_seg_use, const int8_t* d_seg_cap, float* d_total,
+                       int64_t E, unsigned long long* d_count, orx_stream_t stream);
+int orx_via_penalties(const int32_t* d_edge, const int32_t* d_xy, const int32_t* d_z_lo, const int32_t* d_z_hi,
+                      int32_t n, int32_t Nx, int32_t Ny, int32_t segZ, const float* d_col_pres,
+                      const float* d_seg_pres, float col_w, float seg_w, float* d_total, int64_t E,
+                      unsigned long long* d_count, orx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,13 @@ class OrxRouteCfg(C.Structure):
     ]
 
 
+class OrxAcct(C.Structure):
+    _fields_ = [
+        ("E", C.c_int64), ("present", P), ("present_ema", P), ("history", P), ("capacity", P),
+        ("total_cost", P), ("live_static", P), ("canon", P),
+    ]
+
+
 class OrxCsr(C.Structure):
     _fields_ = [
         ("N", C.c_int64), ("E", C.c_int64), ("indptr", P), ("indices", P),
@@ -122,6 +129,20 @@ SIGNATURES = [
     ("orx_sssp_csr", C.c_int, [C.POINTER(OrxCsr), P, C.c_int64, P, P, P, P, C.c_int32, P, P, C.c_int32, P, P,
                                P, P, C.c_int64, P, C.POINTER(OrxRouteCfg), P]),
     ("orx_backtrace_csr", C.c_int, [C.POINTER(OrxCsr), P, P, P, P, C.c_int32, P, P]),
+    ("orx_acct_apply", C.c_int, [C.POINTER(OrxAcct), P, C.c_int64, C.c_int32, C.c_int32, P, P]),
+    ("orx_acct_overuse", C.c_int, [C.POINTER(OrxAcct), P, P, P, P]),
+    ("orx_acct_sum", C.c_int, [C.POINTER(OrxAcct), C.c_int32, P, P]),
+    ("orx_acct_present_ema", C.c_int, [C.POINTER(OrxAcct), C.c_float, C.c_float, P]),
+    ("orx_acct_history", C.c_int, [C.POINTER(OrxAcct), C.c_float, P, C.c_float, C.c_float, C.c_int32, P]),
+    ("orx_acct_costs", C.c_int, [C.POINTER(OrxAcct), P, C.c_float, C.c_float, C.c_float, C.c_float, P, P,
+                                 C.c_int32, P, P, C.c_int32, P, P]),
+    ("orx_acct_begin_live", C.c_int, [C.POINTER(OrxAcct), C.c_float, P, P]),
+    ("orx_acct_refresh_live", C.c_int, [C.POINTER(OrxAcct), P, C.c_int64, C.c_float, P, P, P]),
+    ("orx_acct_from_canonical", C.c_int, [C.POINTER(OrxAcct), P]),
+    ("orx_via_hard_block", C.c_int, [P, P, P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P,
+                                     C.c_int64, P, P]),
+    ("orx_via_penalties", C.c_int, [P, P, P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_float,
+                                    C.c_float, P, C.c_int64, P, P]),
 ]
 
 

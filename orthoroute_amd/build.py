@@ -24,7 +24,8 @@ BUILD_DIR = REPO_DIR / "_build"
 LIB_PATH = PKG_DIR / "liborx_hip.so"
 ORACLE_DIR = REPO_DIR / "oracle"
 
-SOURCES = ["orx_capi.hip", "orx_lattice.hip", "orx_tiles.hip", "orx_csr.hip", "orx_roi.hip", "orx_plan.cpp"]
+SOURCES = ["orx_capi.hip", "orx_lattice.hip", "orx_tiles.hip", "orx_csr.hip", "orx_roi.hip", "orx_accountant.hip",
+           "orx_plan.cpp"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 HIPCC_FLAGS = [
