@@ -13,13 +13,22 @@ Printed: one JSON line.
 - GTEPS.
 - The plain one-net-per-launch form on the same GPU (iteration 1, bit-exact
   check of paths) and the speed-up.
-- ``roofline`` of the route kernel: algorithmic bytes of the label-setting
-  search (12 B per relaxed arc + 16 B per settled node) over the kernel time,
-  measured with HIP events on the stream the waves run on.
+- ``roofline`` of the route launches: algorithmic bytes of the search
+  (12 B per relaxed arc + 16 B per settled node, the label-setting count)
+  over the route time, measured with HIP events on the stream the waves run
+  on.  With the tile engine one launch is one orx_route_tiles call (one
+  iteration's prep / step / drain / post kernels); ``traffic`` is the
+  calibrated FETCH_SIZE + WRITE_SIZE of the same launch from the committed
+  rocprofv3 summary.
+- ``graph``: the timed iterations repeated from the same state with each
+  iteration's launches captured into one hipGraph (capture included).
 - ``cpu_baseline``: the oracle on one host core, on iteration 1 of the same
-  workload.
+  workload, beside the device's own iteration-1 rate.
 - ``sssp_microbench``: 64 full queries on the 1000x1000x12 lattice, one query
-  per launch sequence.
+  per launch sequence; the oracle's heap Dijkstra on the first query is its
+  CPU baseline and must give the same dist bits and parents.
+- ``backplane_plain_vs_batched``: the 400x400x16 backplane, plain one net per
+  launch against batched waves, iterations 1-2 in lockstep.
 - ``serving``: a Poisson stream of routing jobs through the
   continuous-batching server, with throughput and p50/p99 job latency.
 
@@ -49,7 +58,10 @@ if _queues < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"
+TRAFFIC_FILES = {"tiles": REPO / "profiles" / "route_tiles_traffic_uneven_800x800x20.json",
+                 "wave": REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"}
+ROUTE_KERNELS = {"tiles": "orx_route_tiles (tile_prep, tile_step, tile_drain, tile_post kernels of one iteration)",
+                 "wave": "route_wave_kernel"}
 
 
 def _log(msg: str) -> None:
@@ -132,9 +144,13 @@ def copy_probe(mib: int = 256, reps: int = 5) -> None:
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "reps": reps}))
 
 
-def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, tiles: bool = True) -> dict:
+def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, tiles: bool = True,
+                    with_cpu: bool = True) -> dict:
     """Full queries on 1000x1000x12 (costs U[1,4) per physical edge, a seeded
-    5% x10), endpoints uniform on layer 0, one query per launch sequence."""
+    5% x10), endpoints uniform on layer 0, one query per launch sequence.
+    With ``with_cpu``, the oracle (C heap Dijkstra, one core) runs the first
+    query on the same weights: its time is the CPU baseline, and its dist
+    bits and parents must equal the device's."""
     import numpy as np
     import torch
     from orthoroute_amd.lattice import LatticeSpec
@@ -158,6 +174,7 @@ def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, til
     torch.cuda.synchronize()
     times = []
     check = None
+    first = None
     for q in range(n_queries):
         t = time.perf_counter()
         dist, parent, rc = sssp_box(lat, box, int(pairs[q, 0]), int(pairs[q, 1]), full=True, delta=delta,
@@ -165,20 +182,40 @@ def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, til
         times.append(time.perf_counter() - t)
         if q == 0:
             check = _check_path_cost(lat, dist, parent, int(pairs[q, 0]), int(pairs[q, 1]))
+            first = (dist.cpu().numpy(), parent.cpu().numpy())
     st = stats.cpu().numpy()
     total = float(sum(times))
     relaxed, settled = int(st[1]), int(st[0])
     alg = 12 * relaxed + 16 * settled
-    return {
+    engine = ("tile-step search (orx_sssp_box with ORX_F_TILES: tile_fill, tile_step, tile_drain kernels)"
+              if tiles else "grid-wide state machine (orx_sssp_box with ORX_F_GRID: grid_step_kernel)")
+    out = {
         "workload": "sssp_1000x1000x12_full_query", "queries": n_queries, "delta": delta, "block": block,
+        "engine": engine,
         "ms_per_query": total / n_queries * 1e3, "p50_ms": float(np.percentile(times, 50)) * 1e3,
         "gteps": relaxed / total / 1e9, "relaxed_per_query": relaxed / n_queries,
         "settled_per_query": settled / n_queries, "rounds_per_query": int(st[3]) / n_queries,
         "path_cost_check": check,
         "roofline": {"bound": "hbm", "achieved": alg / total / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": alg / total / 1e9 / HBM_PEAK_GBS, "kernel": "grid_step_kernel",
-                     "algorithmic_bytes_per_query": alg / n_queries},
+                     "frac": alg / total / 1e9 / HBM_PEAK_GBS, "kernel": engine,
+                     "algorithmic_bytes_per_query": alg / n_queries,
+                     "timing": "host clock around each synchronous query (all its launches)"},
     }
+    if with_cpu:
+        from oracle.oracle import sssp_box as oracle_sssp_box
+        w = lat.wcost.cpu().numpy()
+        t = time.perf_counter()
+        od, op, _, o_settled, o_relaxed = oracle_sssp_box(spec, w, box, int(pairs[0, 0]), int(pairs[0, 1]),
+                                                          full=True)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": dt * 1e3, "unit": "ms/query", "cores": 1, "kind": "port",
+                               "gteps": o_relaxed / dt / 1e9,
+                               "sample": f"the first query (full 12 M-node box), oracle heap Dijkstra, {dt:.1f} s "
+                                         f"on {_cpu_model()}"}
+        out["same_as_oracle"] = bool(np.array_equal(od.view(np.uint32), first[0].view(np.uint32)) and
+                                     np.array_equal(op, first[1]))
+        out["speedup_vs_cpu"] = dt * 1e3 / out["ms_per_query"]
+    return out
 
 
 def _check_path_cost(lat, dist, parent, src: int, dst: int) -> bool:
@@ -340,14 +377,48 @@ def roi_bench(n: int = 32, samples: int = 4, with_cpu: bool = True) -> dict:
     return out
 
 
-def _traffic():
-    """HBM bytes per route_wave_kernel dispatch from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+def _traffic(engine: str):
+    """HBM bytes per route launch (the tile engine: one orx_route_tiles call,
+    i.e. one iteration's route kernels; the wave engine: one route_wave_kernel
+    dispatch) from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated by a copy of known size), or None."""
     try:
-        t = json.loads(TRAFFIC_FILE.read_text())
-        return float(t["hbm_bytes_per_dispatch"]), t
+        t = json.loads(TRAFFIC_FILES[engine].read_text())
+        return float(t["hbm_bytes_per_launch"]), t
     except (OSError, ValueError, KeyError):
         return None, None
+
+
+def backplane_bench(cfg, iterations: int) -> dict:
+    """The 400x400x16 backplane (1600 long nets + 400 short ones), plain one
+    net per launch against the batched waves, lockstep for ``iterations``
+    iterations with a bit-exact check of every iteration."""
+    import dataclasses
+
+    from orthoroute_amd.netgen import backplane_400
+    nl = backplane_400()
+    res = verify_lockstep(nl, dataclasses.replace(cfg, iterations=iterations), iterations)
+    for r in res["rows"]:
+        r["speedup"] = r["plain_ms"] / r["batched_ms"] if r["batched_ms"] else None
+    return res
+
+
+def graph_leg(dev, snap, steps: int, digest: str) -> dict:
+    """The timed iterations again from the same state, each iteration's
+    route launches captured into one hipGraph and replayed (capture and
+    instantiation inside the timed region, since the wave plan changes every
+    iteration); the result must be the direct launches'."""
+    import torch
+    dev.restore(snap)
+    dev.cfg.collect_stats = False
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = [dev.run_iteration("batched", graph=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"ms_per_step": wall / steps * 1e3, "value": sum(x.routed for x in st) / wall, "unit": "nets/s",
+            "route_ms_per_step": sum(x.route_ms for x in st) / steps,
+            "same_result_as_direct": dev.digest() == digest}
 
 
 def main() -> int:
@@ -375,6 +446,10 @@ def main() -> int:
     ap.add_argument("--portal-steps", type=int, default=2)
     ap.add_argument("--portal-cpu-stride", type=int, default=4)
     ap.add_argument("--graph", action="store_true", help="capture each iteration's wave launches in a hipGraph")
+    ap.add_argument("--no-graph-leg", action="store_true",
+                    help="skip the timed iterations repeated with hipGraph capture")
+    ap.add_argument("--no-backplane", action="store_true")
+    ap.add_argument("--backplane-iterations", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-count", action="store_true",
                     help="skip the counting replay (no algorithmic bytes): for counter passes over the timed kernel")
@@ -440,6 +515,11 @@ def main() -> int:
         counted = [dev.run_iteration("batched") for _ in range(args.steps)]
         if dev.digest() != digest_timed:
             raise RuntimeError("the counting replay did not reproduce the timed iterations")
+    graph = None
+    if snap is not None and not args.no_graph_leg and not args.graph:
+        _log("the timed iterations again with each iteration's launches captured in a hipGraph")
+        graph = graph_leg(dev, snap, args.steps, digest_timed)
+        _log(f"graph: {graph['ms_per_step']:.1f} ms/iteration, same result {graph['same_result_as_direct']}")
     if first is None:
         first = timed[0]
     wall = t1 - t0
@@ -479,7 +559,15 @@ def main() -> int:
     sssp = None
     if not args.no_sssp:
         _log(f"SSSP microbench: {args.sssp_queries} queries on 1000x1000x12")
-        sssp = sssp_microbench(args.sssp_queries, args.sssp_delta, 256, tiles=args.engine == "tiles")
+        sssp = sssp_microbench(args.sssp_queries, args.sssp_delta, 256, tiles=args.engine == "tiles",
+                               with_cpu=not args.no_cpu)
+        torch.cuda.empty_cache()
+
+    backplane = None
+    if not args.no_backplane and args.backplane_iterations > 0:
+        _log(f"backplane 400x400x16: plain vs batched, {args.backplane_iterations} iterations lockstep")
+        backplane = backplane_bench(RouterConfig(delta=delta, block=block, lds_nodes=lds_nodes, engine=args.engine),
+                                    args.backplane_iterations)
         torch.cuda.empty_cache()
 
     serving = None
@@ -512,11 +600,17 @@ def main() -> int:
     if not args.no_cpu:
         _log("CPU baseline (oracle, 1 core)")
         cpu = cpu_baseline(nl, cfg, args.cpu_stride)
+        cpu["device_iteration1"] = {"value": first.routed / first.seconds, "unit": "nets/s",
+                                    "ms": first.seconds * 1e3,
+                                    "note": "the device on the same iteration as the CPU sample (iteration 1, "
+                                            "warm-up, host clock)"}
         _log("CPU baseline (oracle, all cores over disjoint-ROI waves)")
         cpu_all = cpu_baseline_all_cores(nl, cfg)
 
-    traffic, traffic_src = _traffic()
-    n_dispatch = waves
+    traffic, traffic_src = _traffic(args.engine)
+    # one launch: one orx_route_tiles call (an iteration) for the tile engine,
+    # one route_wave_kernel dispatch (a wave) for the wave engine
+    n_dispatch = args.steps if args.engine == "tiles" else waves
     out = {
         "metric": "nets routed/s (batched disjoint-ROI PathFinder iteration)",
         "value": routed / wall,
@@ -533,20 +627,24 @@ def main() -> int:
         "config": {"workload": nl.name, "lattice": [nl.spec.X, nl.spec.Y, nl.spec.L], "nets": nl.n,
                    "roi_margin": 4, "iterations_timed": [timed[0].iteration, timed[-1].iteration],
                    "delta": delta, "block": block, "lds_nodes": lds_nodes, "graph": args.graph,
+                   "engine": args.engine,
                    "parallelism": "1 GPU, disjoint-ROI waves"},
         "gteps": None if relaxed is None else relaxed / route_s / 1e9,
         "waves_per_iteration": waves / args.steps,
         "overuse_last": timed[-1].overuse,
         "hbm_footprint_gb": footprint / 1e9,
         "plain_vs_batched": plain,
+        "graph": graph,
+        "backplane_plain_vs_batched": backplane,
         "p99_job_latency_s": serving["p99_latency_s"] if serving else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "route_wave_kernel",
+                     "kernel": ROUTE_KERNELS[args.engine],
                      "algorithmic_bytes_per_launch": None if alg_bytes is None else alg_bytes / n_dispatch,
                      "kernel_ms_per_launch": route_s * 1e3 / n_dispatch,
                      "launches_per_iteration": n_dispatch / args.steps,
-                     "traffic_source": str(TRAFFIC_FILE.relative_to(REPO)) if traffic_src else None},
+                     "timing": "HIP events on the job's stream around the route launches of each iteration",
+                     "traffic_source": str(TRAFFIC_FILES[args.engine].relative_to(REPO)) if traffic_src else None},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "sssp_microbench": sssp,
