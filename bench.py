@@ -58,6 +58,7 @@ if _queues < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0
+SSSP_TRAFFIC_FILE = REPO / "profiles" / "sssp_tiles_traffic_1000x1000x12.json"
 TRAFFIC_FILES = {"tiles": REPO / "profiles" / "route_tiles_traffic_uneven_800x800x20.json",
                  "wave": REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"}
 ROUTE_KERNELS = {"tiles": "orx_route_tiles (tile_prep, tile_step, tile_drain, tile_post kernels of one iteration)",
@@ -199,8 +200,16 @@ def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, til
         "roofline": {"bound": "hbm", "achieved": alg / total / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / total / 1e9 / HBM_PEAK_GBS, "kernel": engine,
                      "algorithmic_bytes_per_query": alg / n_queries,
-                     "timing": "host clock around each synchronous query (all its launches)"},
+                     "timing": "host clock around each synchronous query (all its launches)",
+                     "traffic": None, "traffic_source": None},
     }
+    if tiles:
+        try:
+            t = json.loads(SSSP_TRAFFIC_FILE.read_text())
+            out["roofline"]["traffic"] = float(t["hbm_bytes_per_query"])
+            out["roofline"]["traffic_source"] = str(SSSP_TRAFFIC_FILE.relative_to(REPO))
+        except (OSError, ValueError, KeyError):
+            pass
     if with_cpu:
         from oracle.oracle import sssp_box as oracle_sssp_box
         w = lat.wcost.cpu().numpy()
@@ -433,6 +442,8 @@ def main() -> int:
     ap.add_argument("--no-sssp", action="store_true")
     ap.add_argument("--sssp-queries", type=int, default=64)
     ap.add_argument("--sssp-delta", type=float, default=16.0)
+    ap.add_argument("--sssp-only", action="store_true",
+                    help="only run the 1000x1000x12 query microbenchmark and print its JSON (profiling runs)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=6)
     ap.add_argument("--serve-rate", type=float, default=2.0)
@@ -475,6 +486,10 @@ def main() -> int:
     from orthoroute_amd.router import DevicePathFinder, RouterConfig
 
     _lib.init_device()
+    if args.sssp_only:
+        print(json.dumps(sssp_microbench(args.sssp_queries, args.sssp_delta, 256, tiles=args.engine == "tiles",
+                                         with_cpu=not args.no_cpu)))
+        return 0
     nl = getattr(netgen, args.workload)()
     tuned = load_tuned(nl)
     if args.verify_lockstep:
