@@ -146,6 +146,9 @@ typedef struct orx_route_cfg {
                             /* seed forms UP:2582-2583, CD:5589-5606)        */
 #define ORX_F_TILES 16      /* orx_sssp_box: tile-step engine (the whole box  */
                             /* relaxed to its fixpoint, see orx_route_tiles)  */
+#define ORX_F_ASYNC 32      /* tile engine, queue form: one persistent launch */
+                            /* per wave visits tiles as they are queued (no   */
+                            /* step launches; h_wave_steps unused)            */
                             /* (bits 16 and up are reserved)                 */
 
 /* stats[]: 0 settled nodes (label-setting count up to the sink; for a

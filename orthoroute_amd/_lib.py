@@ -24,6 +24,7 @@ ORX_F_FULL = 2
 ORX_F_SEED_PEN = 4
 ORX_F_GRID = 8
 ORX_F_TILES = 16
+ORX_F_ASYNC = 32
 ORX_NOALLOC_MAX_NODES = 65536
 
 STATUS_NAMES = {

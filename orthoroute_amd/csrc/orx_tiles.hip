@@ -19,15 +19,31 @@
 // path of many hops costs about one step per tile it crosses instead of one
 // dependent round per hop.
 //
-// Launch discipline: no workgroup ever waits for another.  Step k reads the
-// item list k & 1 and its size cnt[k % 3] (final once step k - 1 ended),
-// appends the next step's items to list (k + 1) & 1 and cnt[(k + 1) % 3], and
-// clears cnt[(k + 2) % 3], which no launch in flight reads.  A tile is queued
-// at most once per list (a bit per tile and list parity).  Two tiles of one
+// Two launch forms drive the visits.
+//
+// Step form: no workgroup ever waits for another.  Step k reads the item
+// list k & 1 and its size cnt[k % 3] (final once step k - 1 ended), appends
+// the next step's items to list (k + 1) & 1 and cnt[(k + 1) % 3], and clears
+// cnt[(k + 2) % 3], which no launch in flight reads.  A tile is queued at
+// most once per list (a bit per tile and list parity).  Two tiles of one
 // step may read each other's boundary keys while they change: any value read
 // is an upper bound of the final key, and a tile that changed its boundary
 // has queued the neighbour again, so the last visit of every tile sees final
 // halo keys.
+//
+// Queue form (ORX_F_ASYNC): one persistent launch per wave.  Its workgroups
+// take tiles from a device ring in ticket order and visit each as soon as it
+// is queued, with no step boundary and no per-step launch.  A state word per
+// tile (dirty / running / seen) keeps every tile queued at most once and
+// never visited by two workgroups at once: a neighbour that changes a halo
+// sets dirty; a tile that is idle is pushed by whoever set dirty; a running
+// one is pushed again by its own visitor when the visit ends.  Key stores
+// are published with a release before the activation that follows them, and
+// a visit starts after an acquire of its ring entry, so a visit sees every
+// halo key that was written before its tile was queued.  A pending counter
+// (tiles queued or running) reaches zero only when the search is done; idle
+// workgroups wait for a ring entry or for that, and every wait is bounded by
+// a wall-clock deadline after which the launch reports an error.
 //
 // The per-net body around the search (rip-up, commit, live re-price;
 // _route_all, unified_pathfinder.py:7949-8430) runs in one prep and one post
@@ -36,6 +52,7 @@
 #include "orx_route.hpp"
 
 #include <chrono>
+#include <cstdlib>
 
 namespace {
 
@@ -46,6 +63,13 @@ constexpr int32_t kMaxL = 32;
 constexpr int32_t kTileBlock = 512;    // lanes of a step workgroup
 constexpr uint32_t kDrainVisits = 1u << 24;   // bound of the single-workgroup drain
 constexpr int32_t kTileSteps = 48;     // steps of one captured graph (multiple of 6: list and counter parity)
+
+// Queue form: tile state bits, ring entry fields (epoch 24 bits | slot 20 |
+// tile 20), and the wall-clock bound of one persistent launch.
+constexpr uint32_t kTsDirty = 1u, kTsRun = 2u, kTsSeen = 4u;
+constexpr uint32_t kRingField = 20;
+constexpr uint32_t kRingMax = 1u << kRingField;   // slots and tiles per slot of one launch
+constexpr int64_t kRunSeconds = 20;
 
 // Layer class of a launch: the step kernels are instantiated for the layer
 // counts of the workloads (6, 8, 12, 16, 20, 24, 32), where a via column has
@@ -65,11 +89,17 @@ __host__ __device__ inline int32_t tile_ty(int32_t L) { return class_ty(layer_cl
 //   w4    the float4 entering weights (slots 0..3), filled by direct
 //         global->LDS loads (the padding column is not loaded);
 //   cm/cp the two halo carries of every planar line (z * kLines + line);
-//   tg/dt a target byte and a dirty byte per node;
-//   pm/vm a byte per planar line / via column: changed since its last scan.
+//   tg    a target byte per node;
+//   mk    the scan marks, bit masks (kMk* offsets): a planar line or via
+//         column is marked when a node of it changed since its last scan.
 struct TileLds {
-    int32_t ty, cells, keys, w4, cm, cp, tg, dt, pm, vm, bytes;
+    int32_t ty, cells, keys, w4, cm, cp, tg, mk, bytes;
 };
+// Mark words: a row ly of the H layers is bit z of mk[kMkHRow + ly], a
+// column lx of the V layers bit z of mk[kMkVCol + lx]; via column (lx, ly)
+// is bit lx of mk[kMkViaRow + ly] (marked by an H-row scan) or bit ly of
+// mk[kMkViaCol + lx] (by a V-column scan).
+constexpr int32_t kMkHRow = 0, kMkVCol = 16, kMkViaRow = 32, kMkViaCol = 48, kMkWords = 64;
 __host__ __device__ inline TileLds tile_lds(int32_t L) {
     TileLds t;
     t.ty = tile_ty(L);
@@ -79,10 +109,8 @@ __host__ __device__ inline TileLds tile_lds(int32_t L) {
     t.cm = t.w4 + 16 * t.cells;
     t.cp = t.cm + 8 * L * kLines;
     t.tg = t.cp + 8 * L * kLines;
-    t.dt = t.tg + t.cells;
-    t.pm = t.dt + t.cells;
-    t.vm = t.pm + L * kLines;
-    t.bytes = (t.vm + kTX * t.ty + 15) / 16 * 16;
+    t.mk = (t.tg + t.cells + 15) / 16 * 16;
+    t.bytes = (t.mk + 4 * kMkWords + 15) / 16 * 16;
     return t;
 }
 
@@ -178,10 +206,63 @@ __device__ __forceinline__ void queue_tile(const TileStep& S, uint32_t s, int64_
     S.item_t[par][at] = t;
 }
 
+// Queue form.  The ring is the first two queue arrays read as u64 entries
+// (S.cap of them); the state word of tile t of the slot at workspace offset
+// off is item_s[1][off + t]; cnt[0] / cnt[1] / cnt[2] are the ring's head
+// and tail tickets and the pending count, cnt[3] the launch epoch.
+__device__ __forceinline__ uint64_t* ring_of(const TileStep& S) { return reinterpret_cast<uint64_t*>(S.item_s[0]); }
+__device__ __forceinline__ uint32_t* tstate_of(const TileStep& S, int64_t off, uint32_t t) {
+    return S.item_s[1] + off + t;
+}
+
+// Append tile t of slot s to the ring (one more pending tile).  A ring
+// entry of this epoch that was never taken means more live entries than the
+// ring holds: reported as an overflow.
+__device__ __forceinline__ void ring_push(const TileStep& S, uint32_t epoch, uint32_t s, uint32_t t) {
+    __hip_atomic_fetch_add(S.cnt + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t at = __hip_atomic_fetch_add(S.cnt + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* e = ring_of(S) + (int64_t)(at % (uint64_t)S.cap);
+    if ((__hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 40) == epoch) atomicOr(S.info + 3, 64ull);
+    __hip_atomic_store(e, ((uint64_t)epoch << 40) | ((uint64_t)s << kRingField) | t, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Ask for a visit of tile t of slot s (its halo changed, or it holds a
+// seed).  The release publishes the caller's earlier key stores to the
+// visit this request leads to.
+__device__ __forceinline__ void ring_activate(const TileStep& S, uint32_t epoch, uint32_t s, int64_t off, uint32_t t) {
+    const uint32_t old =
+        __hip_atomic_fetch_or(tstate_of(S, off, t), kTsDirty, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if ((old & (kTsDirty | kTsRun)) == 0) ring_push(S, epoch, s, t);
+}
+
+// Take the next ticket and wait for its entry; 0 when the search is done (no
+// tile queued or running) or the deadline passed (reported as an error).
+// The wait polls with relaxed loads and ends with one acquire fence.
+__device__ uint64_t ring_pop(const TileStep& S, uint32_t epoch, uint64_t deadline) {
+    const uint32_t at = __hip_atomic_fetch_add(S.cnt + 0, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* e = ring_of(S) + (int64_t)(at % (uint64_t)S.cap);
+    for (uint32_t spin = 0;; ++spin) {
+        const uint64_t v = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((v >> 40) == epoch) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(e, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return v;
+        }
+        if (__hip_atomic_load(S.cnt + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return 0;
+        if ((spin & 63u) == 63u && wall_clock64() > deadline) {
+            atomicOr(S.info + 3, 32ull);
+            return 0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 struct TileShared {
     uint32_t changed[3];
     uint32_t act[4];   // halo of the left / right / lower / upper neighbour changed
     uint32_t first;    // first visit of the tile in this search (its finite keys are unrelaxed seeds)
+    unsigned char zl[64];   // the H layers in order, then (from 32) the V layers
 };
 
 // One relaxation step of a scan: node key k receives the carry c (kept if
@@ -297,34 +378,25 @@ __device__ __forceinline__ uint32_t scan_column(uint64_t* K, const float4* W4, i
                scan_var<ZS, 0, 3u, false, MT>(K, wf, b0, L, tg, ORX_KEY_INF);
 }
 
-// The k-th layer (0-based) of the given orientation: bit z of hmask set
-// (want_h) or clear among the L layers; -1 when there are not that many.
-__device__ __forceinline__ int32_t kth_layer(uint32_t hmask, int32_t L, bool want_h, int32_t k) {
-    for (int32_t z = 0; z < L; ++z) {
-        if ((((hmask >> z) & 1u) != 0) == want_h) {
-            if (k == 0) return z;
-            --k;
-        }
-    }
-    return -1;
-}
-
 // Key-load and write-back loop trip count per lane.
 constexpr int32_t kPerLane = (16 * 16 * 20 + kTileBlock - 1) / kTileBlock;
 
 // The fixpoint phases of a visit.  Planar phase: the lanes of the rows of
 // the H layers first, then (from a multiple of 64, so every wave scans one
 // orientation) the lanes of the columns of the V layers.  A line is scanned
-// when its pm byte is set (a node of it changed in the via phase before, or
-// it holds a seed on the tile's first visit) or, in the first phase, when
-// its halo carry improves its end node; the via phase likewise scans the
-// columns in vm.  Every changed node marks its other line, its dirty byte,
-// and the neighbour whose halo it is.  Returns whether a key changed.
+// when its mark is set (a node of it changed in the via phase before, or it
+// holds a seed on the tile's first visit) or, in the first phase, when its
+// halo carry improves its end node; the via phase likewise scans the marked
+// columns.  A scan that changes nodes marks, with one LDS atomic, the lines
+// of the other phase through them, and the neighbour whose halo it is.  No
+// phase sets the marks it reads, so a scanner clears its own mark bit
+// without a race.  ZL: the H layers in order, then (from 32) the V layers.
+// Returns whether a key changed.
 template <int NT, int LC, bool MT>
 __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t L, int32_t tw,
                                            int32_t th, uint64_t* K, const float4* W4, const uint64_t* CM,
-                                           const uint64_t* CP, const unsigned char* TG, unsigned char* DT,
-                                           unsigned char* PM, unsigned char* VM, TileShared& sh) {
+                                           const uint64_t* CP, const unsigned char* TG, uint32_t* MK,
+                                           const unsigned char* ZL, TileShared& sh) {
     constexpr int32_t TY = class_ty(LC);
     constexpr int32_t ZS = TY * kPitch;
     const uint32_t tid = threadIdx.x;
@@ -336,17 +408,20 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
 #pragma unroll 1
         for (int32_t p = (int32_t)tid; p < pl; p += NT) {
             const bool horiz = p < hl;
-            const int32_t z = kth_layer(S.hmask, L, horiz, (horiz ? p : p - hl) >> 4);
-            const int32_t l = (horiz ? p : p - hl) & 15;
+            const int32_t q = horiz ? p : p - hl, k = q >> 4, l = q & 15;
             const int32_t n = horiz ? tw : th;
-            if (z < 0 || l >= (horiz ? th : tw)) continue;
+            if (k >= (horiz ? nh : L - nh) || l >= (horiz ? th : tw)) continue;
+            const int32_t z = ZL[(horiz ? 0 : 32) + k];
             const int32_t li = z * kLines + l;
             const int32_t b0 = horiz ? (z * TY + l) * kPitch : z * ZS + l;
             const int32_t st = horiz ? 1 : kPitch;
+            uint32_t* mw = MK + (horiz ? kMkHRow : kMkVCol) + l;
+            const uint32_t bit = 1u << z;
             const uint64_t cmi = CM[li], cpi = CP[li];
-            const bool need = PM[li] != 0 || (j == 0 && (cmi < K[b0] || cpi < K[b0 + (n - 1) * st]));
+            const bool marked = (*mw & bit) != 0;
+            const bool need = marked || (j == 0 && (cmi < K[b0] || cpi < K[b0 + (n - 1) * st]));
             if (!need) continue;
-            PM[li] = 0;
+            if (marked) atomicAnd(mw, ~bit);
             uint32_t tg = 0;
             if (MT)
                 for (int32_t i = 0; i < n; ++i) tg |= (uint32_t)TG[b0 + i * st] << i;
@@ -355,18 +430,16 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
             if (cm) {
                 if (cm & 1u) sh.act[horiz ? 0 : 2] = 1u;
                 if ((cm >> (n - 1)) & 1u) sh.act[horiz ? 1 : 3] = 1u;
-                for (uint32_t m = cm; m; m &= m - 1) {
-                    const int32_t i = __ffs(m) - 1;
-                    DT[b0 + i * st] = 1;
-                    VM[horiz ? l * kTX + i : i * kTX + l] = 1;
-                }
+                // H row l: via columns (lx in cm, l); V column l: (l, ly in cm)
+                atomicOr(MK + (horiz ? kMkViaRow : kMkViaCol) + l, cm);
                 chg |= cm;
             }
         }
     } else if ((int32_t)tid < kTX * TY) {
         const int32_t lx = (int32_t)tid & (kTX - 1), ly = (int32_t)tid >> 4;
-        if (lx < tw && ly < th && VM[tid]) {
-            VM[tid] = 0;
+        if (lx < tw && ly < th && (((MK[kMkViaRow + ly] >> lx) | (MK[kMkViaCol + lx] >> ly)) & 1u)) {
+            atomicAnd(MK + kMkViaRow + ly, ~(1u << lx));
+            atomicAnd(MK + kMkViaCol + lx, ~(1u << ly));
             const int32_t b0 = ly * kPitch + lx;
             uint32_t tg = 0;
             if (MT)
@@ -377,11 +450,9 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
                 if (lx == tw - 1 && (cm & hm)) sh.act[1] = 1u;
                 if (ly == 0 && (cm & ~hm)) sh.act[2] = 1u;
                 if (ly == th - 1 && (cm & ~hm)) sh.act[3] = 1u;
-                for (uint32_t m = cm; m; m &= m - 1) {
-                    const int32_t z = __ffs(m) - 1;
-                    DT[b0 + z * ZS] = 1;
-                    PM[z * kLines + (((hm >> z) & 1u) ? ly : lx)] = 1;
-                }
+                // changed nodes of H layers lie on row ly, of V layers on column lx
+                if (cm & hm) atomicOr(MK + kMkHRow + ly, cm & hm);
+                if (cm & ~hm) atomicOr(MK + kMkVCol + lx, cm & ~hm);
                 chg = cm;
             }
         }
@@ -392,10 +463,11 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
 // One visit of tile t of slot s: load, local fixpoint, write-back, activation.
 // Every lane of the workgroup calls it; it ends with a barrier.  The global
 // loads are issued together (one memory wait per visit): the weights go
-// straight into LDS, the keys through registers.
-template <int NT, int LC>
+// straight into LDS, the keys through registers.  AQ: the queue form (pin,
+// pout and cnt_out unused; epoch tags the activations).
+template <int NT, int LC, bool AQ>
 __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t pin, uint32_t pout, uint32_t* cnt_out,
-                           unsigned char* lds, TileShared& sh) {
+                           uint32_t epoch, unsigned char* lds, TileShared& sh) {
     constexpr int32_t TY = class_ty(LC);
     constexpr int32_t ZS = TY * kPitch;   // node-index stride of one layer
     static_assert(NT * kPerLane >= kTX * 16 * 20 && NT * kPerLane >= kTX * 10 * 32, "the key loop covers a tile");
@@ -414,9 +486,7 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
     uint64_t* CM = reinterpret_cast<uint64_t*>(lds + TL.cm);
     uint64_t* CP = reinterpret_cast<uint64_t*>(lds + TL.cp);
     unsigned char* TG = lds + TL.tg;
-    unsigned char* DT = lds + TL.dt;
-    unsigned char* PM = lds + TL.pm;
-    unsigned char* VM = lds + TL.vm;
+    uint32_t* MK = reinterpret_cast<uint32_t*>(lds + TL.mk);
     const float4* w4 = reinterpret_cast<const float4*>(S.wcost);
     const int64_t XY = S.XY;
     const int32_t span = kTX * TY * L;
@@ -426,10 +496,25 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
 
     uint32_t seen_w = 0;
     if (tid == 0) {
-        __hip_atomic_fetch_and(S.marks[pin] + G.off / 32 + (t >> 5), ~tbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        seen_w = __hip_atomic_fetch_or(seen, tbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (AQ) {
+            // taken: running and seen, dirty cleared (a halo change from now
+            // on asks for another visit)
+            seen_w = (__hip_atomic_exchange(tstate_of(S, G.off, t), kTsRun | kTsSeen, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT) & kTsSeen)
+                         ? tbit
+                         : 0u;
+        } else {
+            __hip_atomic_fetch_and(S.marks[pin] + G.off / 32 + (t >> 5), ~tbit, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            seen_w = __hip_atomic_fetch_or(seen, tbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         sh.changed[0] = sh.changed[1] = sh.changed[2] = 0;
         sh.act[0] = sh.act[1] = sh.act[2] = sh.act[3] = 0;
+        int32_t nh = 0, nv = 0;
+        for (int32_t z = 0; z < L; ++z) {
+            if ((S.hmask >> z) & 1u) sh.zl[nh++] = (unsigned char)z;
+            else sh.zl[32 + nv++] = (unsigned char)z;
+        }
     }
     // 1. every global load of the visit at once: the weights straight into
     // LDS (16 bytes per lane, each wave filling 64 consecutive node indices;
@@ -486,16 +571,14 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
                 hw[1] = S.wcost[4 * (glay + (int64_t)(B.y0 + ay + th) * S.X) + 1];
             }
         }
-        PM[tid] = 0;
     }
-    if ((int32_t)tid < kTX * TY) VM[tid] = 0;
+    if ((int32_t)tid < kMkWords) MK[tid] = 0u;
 #pragma unroll
     for (int u = 0; u < kPerLane; ++u) {
         const int32_t i = (int32_t)tid + u * NT;
         if (i < span) {
             const int32_t li = (i >> 4) * kPitch + (i & (kTX - 1));
             if (multi) TG[li] = 0;
-            DT[li] = 0;
             K[li] = lk[u];
         }
     }
@@ -516,8 +599,11 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
             const int32_t i = (int32_t)tid + u * NT;
             if (i < span && lk[u] != ORX_KEY_INF) {
                 const int32_t lx = i & (kTX - 1), q = i >> 4, z = q / TY, ly = q - z * TY;
-                PM[z * kLines + (((S.hmask >> z) & 1u) ? ly : lx)] = 1;
-                VM[ly * kTX + lx] = 1;
+                if ((S.hmask >> z) & 1u)
+                    atomicOr(MK + kMkHRow + ly, 1u << z);
+                else
+                    atomicOr(MK + kMkVCol + lx, 1u << z);
+                atomicOr(MK + kMkViaRow + ly, 1u << lx);
             }
         }
         __syncthreads();
@@ -552,8 +638,8 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
     // visit, and only the carries are new).
     int32_t j = 0;
     for (;; ++j) {
-        const bool chg = multi ? tile_phase<NT, LC, true>(S, j, L, tw, th, K, W4, CM, CP, TG, DT, PM, VM, sh)
-                               : tile_phase<NT, LC, false>(S, j, L, tw, th, K, W4, CM, CP, TG, DT, PM, VM, sh);
+        const bool chg = multi ? tile_phase<NT, LC, true>(S, j, L, tw, th, K, W4, CM, CP, TG, MK, sh.zl, sh)
+                               : tile_phase<NT, LC, false>(S, j, L, tw, th, K, W4, CM, CP, TG, MK, sh.zl, sh);
         if (chg) sh.changed[j % 3] = 1u;
         __syncthreads();
         const bool any = sh.changed[j % 3] != 0;
@@ -565,14 +651,23 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
         }
     }
     // 4. activation of the neighbours whose halo changed (lanes 0-3), while
-    // every lane writes back what improved
-    if (tid < 4 && sh.act[tid]) {
-        const int32_t nx = tx + (tid == 0 ? -1 : (tid == 1 ? 1 : 0));
-        const int32_t ny = tyi + (tid == 2 ? -1 : (tid == 3 ? 1 : 0));
-        if (nx >= 0 && nx < G.ntx && ny >= 0 && ny < G.nty)
-            queue_tile(S, s, G.off, (uint32_t)(ny * G.ntx + nx), pout, cnt_out);
-    }
-    // the write-back recomputes its indices from an opaque copy of the lane
+    // every lane writes back what improved (step form); the queue form
+    // writes back first and activates after a release of every lane's stores
+    auto activate = [&]() {
+        if (tid < 4 && sh.act[tid]) {
+            const int32_t nx = tx + (tid == 0 ? -1 : (tid == 1 ? 1 : 0));
+            const int32_t ny = tyi + (tid == 2 ? -1 : (tid == 3 ? 1 : 0));
+            if (nx >= 0 && nx < G.ntx && ny >= 0 && ny < G.nty) {
+                if constexpr (AQ)
+                    ring_activate(S, epoch, s, G.off, (uint32_t)(ny * G.ntx + nx));
+                else
+                    queue_tile(S, s, G.off, (uint32_t)(ny * G.ntx + nx), pout, cnt_out);
+            }
+        }
+    };
+    if constexpr (!AQ) activate();
+    // write-back of the keys that changed (against the loaded ones, still in
+    // registers); the indices are recomputed from an opaque copy of the lane
     // id, so the compiler does not keep the load loop's indices alive
     // through the fixpoint
     uint32_t wtid = tid;
@@ -582,11 +677,33 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
         const int32_t i = (int32_t)wtid + u * NT;
         if (i < span) {
             const int32_t lx = i & (kTX - 1), q = i >> 4;
-            const int32_t li = q * kPitch + lx;
-            if (DT[li]) {
+            const uint64_t kn = K[q * kPitch + lx];
+            if (kn != lk[u]) {
                 const int32_t z = q / TY, ly = q - z * TY;
-                gstore_key(keys + (uint32_t)((z * B.h + (ay + ly)) * B.w + (ax + lx)), K[li]);
+                gstore_key(keys + (uint32_t)((z * B.h + (ay + ly)) * B.w + (ax + lx)), kn);
             }
+        }
+    }
+    if constexpr (AQ) {
+        // the barrier orders every lane's key stores before the activations'
+        // release (agent scope) in lanes 0-3
+        __syncthreads();
+        activate();
+        // every push of lanes 0-3 (pending raised) before lane 0 drops this
+        // run's count: pending may not touch zero while a push is under way
+        __syncthreads();
+        if (tid == 0) {
+            // the visit ends: idle again, unless a halo changed meanwhile --
+            // then this tile goes back on the ring (pending is raised by the
+            // push before this run's own count is dropped)
+            uint32_t* st = tstate_of(S, G.off, t);
+            uint32_t expect = kTsRun | kTsSeen;
+            if (!__hip_atomic_compare_exchange_strong(st, &expect, kTsSeen, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_exchange(st, kTsDirty | kTsSeen, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                ring_push(S, epoch, s, t);
+            }
+            __hip_atomic_fetch_sub(S.cnt + 2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     if (tid == 0 && (S.flags & ORX_F_STATS)) {
@@ -596,16 +713,27 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
     __syncthreads();
 }
 
-// Keys of [0, n) to +inf, and the step counters / info cleared.
+// Keys of [0, n) to +inf, the step counters / info cleared and the launch
+// epoch of the queue form advanced (1 .. 2^24 - 1).  Queue form: the ring's
+// first n entries (the ring of a launch over n workspace nodes) cleared too,
+// so no entry left by another engine's use of the queue arrays can pass for
+// one of this launch.
 __global__ __launch_bounds__(256) void tile_fill_kernel(TileStep S, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     ulonglong2* k2 = reinterpret_cast<ulonglong2*>(S.keys);
     const ulonglong2 inf2 = make_ulonglong2(ORX_KEY_INF, ORX_KEY_INF);
     for (int64_t i = t; i < n / 2; i += stride) k2[i] = inf2;
+    if (S.flags & ORX_F_ASYNC) {
+        ulonglong2* r2 = reinterpret_cast<ulonglong2*>(ring_of(S));
+        const ulonglong2 zero2 = make_ulonglong2(0ull, 0ull);
+        for (int64_t i = t; i < n / 2; i += stride) r2[i] = zero2;
+        if (t == 0 && (n & 1)) ring_of(S)[n - 1] = 0ull;
+    }
     if (t == 0) {
         if (n & 1) S.keys[n - 1] = ORX_KEY_INF;
         S.cnt[0] = S.cnt[1] = S.cnt[2] = 0;
+        S.cnt[3] = S.cnt[3] % 0xFFFFFFu + 1u;
         S.info[0] = S.info[1] = S.info[2] = S.info[3] = 0;
     }
 }
@@ -671,13 +799,20 @@ __global__ __launch_bounds__(BLOCK) void tile_prep_kernel(TileArgs A, int32_t ri
         if (rsrc == UINT32_MAX) st = ORX_E_ARG;
         if (wave && global_to_roi(g, B, nets.dst[k]) == UINT32_MAX) st = ORX_E_ARG;
     }
+    const int32_t ntiles = G.ntx * G.nty;
+    const bool async = (S.flags & ORX_F_ASYNC) != 0;
+    if (st == ORX_OK && async && (uint32_t)ntiles > kRingMax) st = ORX_E_ARG;
     if (tid == 0 && wave) nets.status[k] = st;
     if (st != ORX_OK) return;
-    const int32_t ntiles = G.ntx * G.nty;
-    for (int32_t i = (int32_t)tid; i < (ntiles + 31) / 32; i += BLOCK) {
-        S.marks[0][G.off / 32 + i] = 0u;
-        S.marks[1][G.off / 32 + i] = 0u;
-        S.seen[G.off / 32 + i] = 0u;
+    const uint32_t epoch = gload_u32(S.cnt + 3);
+    if (async) {
+        for (int32_t i = (int32_t)tid; i < ntiles; i += BLOCK) *tstate_of(S, G.off, (uint32_t)i) = 0u;
+    } else {
+        for (int32_t i = (int32_t)tid; i < (ntiles + 31) / 32; i += BLOCK) {
+            S.marks[0][G.off / 32 + i] = 0u;
+            S.marks[1][G.off / 32 + i] = 0u;
+            S.seen[G.off / 32 + i] = 0u;
+        }
     }
     __syncthreads();
     uint64_t* keys = S.keys + G.off;
@@ -689,7 +824,10 @@ __global__ __launch_bounds__(BLOCK) void tile_prep_kernel(TileArgs A, int32_t ri
     if (!multi) {
         if (tid == 0) {
             keys[rsrc] = kpack(0.0f, 0u, ORX_ROOT_SLOT);
-            queue_tile(S, s, G.off, tile_of(rsrc), 0u, &S.cnt[0]);
+            if (async)
+                ring_activate(S, epoch, s, G.off, tile_of(rsrc));
+            else
+                queue_tile(S, s, G.off, tile_of(rsrc), 0u, &S.cnt[0]);
         }
         return;
     }
@@ -707,7 +845,11 @@ __global__ __launch_bounds__(BLOCK) void tile_prep_kernel(TileArgs A, int32_t ri
     __syncthreads();
     for (int32_t i = (int32_t)tid; i < ns; i += BLOCK) {
         const uint32_t rs = global_to_roi(g, B, nets.seed_node[s0 + i]);
-        if (rs != UINT32_MAX) queue_tile(S, s, G.off, tile_of(rs), 0u, &S.cnt[0]);
+        if (rs == UINT32_MAX) continue;
+        if (async)
+            ring_activate(S, epoch, s, G.off, tile_of(rs));
+        else
+            queue_tile(S, s, G.off, tile_of(rs), 0u, &S.cnt[0]);
     }
 }
 
@@ -724,8 +866,8 @@ __global__ __launch_bounds__(kTileBlock) void tile_step_kernel(TileStep S, uint3
     }
     const uint32_t pin = k & 1u, pout = pin ^ 1u;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
-        tile_visit<kTileBlock, LC>(S, S.item_s[pin][i], S.item_t[pin][i], pin, pout, S.cnt + (k + 1) % 3,
-                                   reinterpret_cast<unsigned char*>(lds_mem), sh);
+        tile_visit<kTileBlock, LC, false>(S, S.item_s[pin][i], S.item_t[pin][i], pin, pout, S.cnt + (k + 1) % 3, 0u,
+                                          reinterpret_cast<unsigned char*>(lds_mem), sh);
 }
 
 // The tail of a step budget: one workgroup runs the remaining steps until no
@@ -747,13 +889,34 @@ __global__ __launch_bounds__(kTileBlock) void tile_drain_kernel(TileStep S, uint
         if (threadIdx.x == 0) S.cnt[(k + 2) % 3] = 0;
         const uint32_t pin = k & 1u, pout = pin ^ 1u;
         for (uint32_t i = 0; i < n; ++i)
-            tile_visit<kTileBlock, LC>(S, S.item_s[pin][i], S.item_t[pin][i], pin, pout, S.cnt + (k + 1) % 3,
-                                       reinterpret_cast<unsigned char*>(lds_mem), sh);
+            tile_visit<kTileBlock, LC, false>(S, S.item_s[pin][i], S.item_t[pin][i], pin, pout,
+                                              S.cnt + (k + 1) % 3, 0u, reinterpret_cast<unsigned char*>(lds_mem),
+                                              sh);
         visits += n;
         __threadfence();
         __syncthreads();
     }
     if (threadIdx.x == 0 && visits) atomicAdd(S.info + 2, (unsigned long long)visits);
+}
+
+// Queue form: the persistent launch of one wave (or one search).  Lane 0
+// takes a ticket and waits for its ring entry; the workgroup visits that
+// tile; until no tile is queued or running, or the deadline.
+template <int LC>
+__global__ __launch_bounds__(kTileBlock) void tile_run_kernel(TileStep S, uint64_t deadline_ticks) {
+    ORX_DYNAMIC_LDS(uint64_t, lds_mem);
+    ORX_SHARED(TileShared, sh);
+    ORX_SHARED(uint64_t, s_item);
+    const uint32_t epoch = gload_u32(S.cnt + 3);
+    const uint64_t deadline = wall_clock64() + deadline_ticks;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = ring_pop(S, epoch, deadline);
+        __syncthreads();
+        const uint64_t v = s_item;
+        if (v == 0) break;
+        tile_visit<kTileBlock, LC, true>(S, (uint32_t)(v >> kRingField) & (kRingMax - 1u), (uint32_t)v & (kRingMax - 1u),
+                                         0u, 0u, nullptr, epoch, reinterpret_cast<unsigned char*>(lds_mem), sh);
+    }
 }
 
 // The tail of the per-net body, one workgroup per slot: the chosen target
@@ -780,6 +943,10 @@ __global__ __launch_bounds__(BLOCK) void tile_post_kernel(TileArgs A) {
     const bool full = (S.flags & ORX_F_FULL) != 0;
     const bool multi = nets.seed_ptr && nets.seed_ptr[k + 1] > nets.seed_ptr[k];
     int32_t st = nets.status[k];
+    // a list overflow, broken state or missed deadline of this wave's search
+    // leaves its keys incomplete: no net of the wave commits
+    if (st == ORX_OK && __hip_atomic_load(S.info + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        st = ORX_E_OVERFLOW;
     if (tid == 0) { ctl.settled = 0; ctl.relaxed = 0; ctl.path_len = 0; }
     int32_t* path = nets.path_arena + nets.path_off[k];
     const int32_t cap = nets.path_cap[k];
@@ -957,6 +1124,7 @@ int check_tile_ws(const orx_workspace* ws) {
         return orx_host::fail(ORX_E_ARG, "the tile engine needs a workspace of at least 1024 nodes");
     if ((uintptr_t)ws->keys & 15u) return orx_host::fail(ORX_E_ARG, "workspace keys must be 16-byte aligned");
     if ((uintptr_t)ws->bitmaps & 7u) return orx_host::fail(ORX_E_ARG, "workspace bitmaps must be 8-byte aligned");
+    if ((uintptr_t)ws->queues & 7u) return orx_host::fail(ORX_E_ARG, "workspace queues must be 8-byte aligned");
     return ORX_OK;
 }
 
@@ -964,6 +1132,7 @@ template <int LC>
 int tile_lds_ready_lc(size_t b) {
     int rc = set_lds<tile_step_kernel<LC>>(b);
     if (!rc) rc = set_lds<tile_drain_kernel<LC>>(b);
+    if (!rc) rc = set_lds<tile_run_kernel<LC>>(b);
     return rc;
 }
 
@@ -994,6 +1163,45 @@ void launch_step(const TileStep& S, uint32_t k, int grid, size_t lds, hipStream_
     }
 }
 
+// Wall-clock ticks of the persistent launch's deadline: kRunSeconds, or
+// ORX_TILE_RUN_SECONDS from the environment (read once; test builds use it to
+// stretch the bound for slow emulation or to force the error path).
+uint64_t run_deadline_ticks() {
+    static uint64_t ticks = 0;
+    if (ticks == 0) {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        double secs = (double)kRunSeconds;
+        if (const char* e = std::getenv("ORX_TILE_RUN_SECONDS")) {
+            const double v = std::atof(e);
+            if (v > 0.0 && v < 1e6) secs = v;
+        }
+        ticks = std::max<uint64_t>(1, (uint64_t)((double)khz * 1000.0 * secs));
+    }
+    return ticks;
+}
+
+// Workgroups of a persistent launch: the chip's, or fewer when the wave has
+// fewer tiles (every slot's tiles, bounded by its share of the extent).
+int run_grid(int32_t L, int32_t K, int64_t extent) {
+    const int64_t tile_nodes = (int64_t)kTX * tile_ty(L) * L;
+    const int64_t tiles = (int64_t)K + 2 * ((extent + tile_nodes - 1) / tile_nodes);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(tile_grid(L), tiles));
+}
+
+void launch_run(const TileStep& S, int grid, size_t lds, hipStream_t s) {
+    switch (layer_class(S.L)) {
+#define ORX_CASE(LC)                                                                                 \
+    case LC:                                                                                         \
+        hipLaunchKernelGGL(tile_run_kernel<LC>, dim3(grid), dim3(kTileBlock), lds, s, S, run_deadline_ticks()); \
+        break;
+        ORX_TILE_CLASSES(ORX_CASE)
+#undef ORX_CASE
+    }
+}
+
 void launch_drain(const TileStep& S, uint32_t k0, size_t lds, hipStream_t s) {
     switch (layer_class(S.L)) {
 #define ORX_CASE(LC) \
@@ -1011,13 +1219,20 @@ int launch_tile_wave(TileArgs A, const int32_t* d_wave_nets, const int64_t* d_ws
     A.S.slot_off = d_ws_off;
     A.K = K;
     A.extent = std::min<int64_t>(ws_extent, A.S.cap - kTileReserve);
+    // queue form: the ring holds as many entries as the wave has workspace
+    // nodes (at least one per tile), cleared by the fill
+    if (A.S.flags & ORX_F_ASYNC) A.S.cap = std::max<int64_t>(A.extent, 2);
     const size_t lds = (size_t)tile_lds(A.S.L).bytes;
     const int fill_grid = (int)std::max<int64_t>(1, std::min<int64_t>(device_cus() * 8, (A.extent / 2 + 255) / 256));
     hipLaunchKernelGGL(tile_fill_kernel, dim3(fill_grid), dim3(256), 0, s, A.S, A.extent);
     hipLaunchKernelGGL(tile_prep_kernel<256>, dim3(K), dim3(256), 0, s, A, 1);
-    const int grid = tile_grid(A.S.L);
-    for (int32_t k = 0; k < steps; ++k) launch_step(A.S, (uint32_t)k, grid, lds, s);
-    launch_drain(A.S, (uint32_t)steps, lds, s);
+    if (A.S.flags & ORX_F_ASYNC) {
+        launch_run(A.S, run_grid(A.S.L, K, A.extent), lds, s);
+    } else {
+        const int grid = tile_grid(A.S.L);
+        for (int32_t k = 0; k < steps; ++k) launch_step(A.S, (uint32_t)k, grid, lds, s);
+        launch_drain(A.S, (uint32_t)steps, lds, s);
+    }
     switch (block) {
     case 256: hipLaunchKernelGGL(tile_post_kernel<256>, dim3(K), dim3(256), 0, s, A); break;
     case 512: hipLaunchKernelGGL(tile_post_kernel<512>, dim3(K), dim3(512), 0, s, A); break;
@@ -1057,6 +1272,7 @@ int sssp_box_tiles(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, in
     const int64_t R = (int64_t)w * h * lat->L;
     if (R > A.extent) return orx_host::fail(ORX_E_ARG, "workspace smaller than the ROI plus the tile engine's reserve");
     A.extent = R;
+    if (cfg->flags & ORX_F_ASYNC) A.S.cap = std::max<int64_t>(R, 2);
     const bool full = dst < 0 || (cfg->flags & ORX_F_FULL);
     uint32_t rdst = 0;
     if (dst >= 0) {
@@ -1087,15 +1303,21 @@ int sssp_box_tiles(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, in
         e = hipGetLastError();
     }
     static_assert(kTileSteps % 6 == 0, "list (mod 2) and counter (mod 3) parity must carry over between replays");
-    if (e == hipSuccess) e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
+    const bool async = (cfg->flags & ORX_F_ASYNC) != 0;
+    if (async && e == hipSuccess) {
+        // queue form: one persistent launch runs the search to its end
+        launch_run(A.S, run_grid(lat->L, 1, R), lds, ps);
+        e = hipGetLastError();
+    }
+    if (!async && e == hipSuccess) e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
+    if (!async && e == hipSuccess) {
         for (int i = 0; i < kTileSteps; ++i) launch_step(A.S, (uint32_t)i, grid, lds, ps);
         e = hipStreamEndCapture(ps, &graph);
     }
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&h_cnt, 2 * sizeof(uint32_t), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    if (!async && e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (!async && e == hipSuccess) e = hipHostMalloc((void**)&h_cnt, 2 * sizeof(uint32_t), hipHostMallocDefault);
+    if (!async && e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (!async && e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
     // after each replay the next step's list size is cnt[0] (kTileSteps % 3
     // == 0); replays run two deep so the device does not idle through the
     // host's round trip
@@ -1105,8 +1327,8 @@ int sssp_box_tiles(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, in
         if (x == hipSuccess) x = hipEventRecord(ev[slot], ps);
         return x;
     };
-    if (e == hipSuccess) e = replay(0);
-    if (e == hipSuccess) {
+    if (!async && e == hipSuccess) e = replay(0);
+    if (!async && e == hipSuccess) {
         const auto t0 = std::chrono::steady_clock::now();
         for (int it = 0;; ++it) {
             const int cur = it & 1;
@@ -1171,6 +1393,8 @@ int orx_route_tiles(const orx_lattice* lat, const orx_params* p, const orx_nets*
             return orx_host::fail(ORX_E_ARG, "wave workspace extent outside [0, ws_nodes - 512]");
         if (h_wave_steps[i] < 0 || h_wave_steps[i] > 65536)
             return orx_host::fail(ORX_E_ARG, "wave step budget outside [0, 65536]");
+        if ((cfg->flags & ORX_F_ASYNC) && h_wave_ptr[i + 1] - h_wave_ptr[i] > (int32_t)kRingMax)
+            return orx_host::fail(ORX_E_ARG, "the queue form takes at most 2^20 nets per wave");
     }
     rc = tile_lds_ready(lat->L);
     if (rc) return rc;

@@ -64,6 +64,7 @@ class RouterConfig:
     adapt_delta: bool = True          # widen delta x4 after an iteration of many thin threshold phases
     delta_max: float = 1048576.0
     engine: str = "tiles"             # "tiles": tile-step search (orx_route_tiles); "wave": one workgroup per net
+    tile_queue: bool = False          # tile engine, queue form: one persistent launch per wave (ORX_F_ASYNC)
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -496,7 +497,8 @@ class DevicePathFinder:
 
     def route_cfg(self, flags: int = 0) -> _lib.OrxRouteCfg:
         c = self.cfg
-        fl = flags | (_lib.ORX_F_STATS if c.collect_stats else 0)
+        fl = flags | (_lib.ORX_F_STATS if c.collect_stats else 0) | \
+            (_lib.ORX_F_ASYNC if c.engine == "tiles" and c.tile_queue else 0)
         return _lib.OrxRouteCfg(self.delta, int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr())
 
     def _adapt_delta(self, st: "IterationStats") -> None:

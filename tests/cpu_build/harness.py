@@ -156,13 +156,14 @@ SSSP_DRIVER = HERE / "_out" / "sssp_driver"
 
 
 def sssp_box_host_threads(spec, wcost: np.ndarray, box, src: int, dst: int, workdir: Path, grid: bool = True,
-                          delta: float = 2.0, block: int = 256, timeout: float = 600.0):
-    """One orx_sssp_box query through the host-thread build: (dist, parent, stats)."""
+                          delta: float = 2.0, block: int = 256, timeout: float = 600.0, flags: int = 0):
+    """One orx_sssp_box query through the host-thread build: (dist, parent,
+    stats).  flags: extra ORX_F_* bits (ORX_F_TILES, ORX_F_ASYNC)."""
     d = Path(workdir)
     d.mkdir(parents=True, exist_ok=True)
     x0, y0, w, h = (int(v) for v in box)
     meta = {"X": spec.X, "Y": spec.Y, "L": spec.L, "hmask": spec.hmask, "x0": x0, "y0": y0, "w": w, "h": h,
-            "src": src, "dst": dst, "flags": (8 if grid else 0) | 1 | (2 if dst < 0 else 0), "block": block,
+            "src": src, "dst": dst, "flags": (8 if grid else 0) | 1 | (2 if dst < 0 else 0) | flags, "block": block,
             "delta": delta}
     with open(d / "meta.txt", "w") as f:
         for k, v in meta.items():

@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <atomic>
+#include <chrono>
 #include <barrier>
 #include <cmath>
 #include <cstdlib>
@@ -256,6 +257,22 @@ T __hip_atomic_fetch_min(T* p, T v, int, int) {
 template <typename T>
 T __hip_atomic_fetch_or(T* p, T v, int, int) { return std::atomic_ref<T>(*p).fetch_or(v); }
 template <typename T>
+T __hip_atomic_fetch_add(T* p, T v, int, int) { return std::atomic_ref<T>(*p).fetch_add(v); }
+template <typename T>
+T __hip_atomic_fetch_sub(T* p, T v, int, int) { return std::atomic_ref<T>(*p).fetch_sub(v); }
+template <typename T>
+T __hip_atomic_exchange(T* p, T v, int, int) { return std::atomic_ref<T>(*p).exchange(v); }
+template <typename T>
+bool __hip_atomic_compare_exchange_strong(T* p, T* expected, T desired, int, int, int) {
+    return std::atomic_ref<T>(*p).compare_exchange_strong(*expected, desired);
+}
+#define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
+// the device wall clock: 100 MHz, as hipDeviceAttributeWallClockRate reports
+inline uint64_t wall_clock64() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch()).count() / 10u;
+}
+template <typename T>
 T __hip_atomic_fetch_and(T* p, T v, int, int) { return std::atomic_ref<T>(*p).fetch_and(v); }
 template <typename T, typename U>
 T atomicAdd(T* p, U v) { return std::atomic_ref<T>(*p).fetch_add((T)v); }
@@ -263,6 +280,8 @@ template <typename T, typename U>
 T atomicMin(T* p, U v) { return __hip_atomic_fetch_min(p, (T)v, 0, 0); }
 template <typename T, typename U>
 T atomicOr(T* p, U v) { return std::atomic_ref<T>(*p).fetch_or((T)v); }
+template <typename T, typename U>
+T atomicAnd(T* p, U v) { return std::atomic_ref<T>(*p).fetch_and((T)v); }
 template <typename T, typename U, typename V>
 T atomicCAS(T* p, U expected, V desired) {
     T cur = (T)expected;
@@ -282,6 +301,7 @@ typedef int hipError_t;
 typedef void* hipStream_t;
 #define hipSuccess 0
 enum { hipFuncAttributeMaxDynamicSharedMemorySize = 8, hipDeviceAttributeMultiprocessorCount = 16,
+       hipDeviceAttributeWallClockRate = 17,
        hipMemcpyDeviceToHost = 2 };
 struct hipDeviceProp_t { char gcnArchName[256]; };
 inline const char* hipGetErrorString(hipError_t) { return "hipcpu error"; }
@@ -289,7 +309,10 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) { strcpy(p->gcnArchName, "gfx950:cpu"); return 0; }
-inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = 4; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int* v, int a, int) {
+    *v = a == hipDeviceAttributeWallClockRate ? 100000 : 4;
+    return hipSuccess;
+}
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess; }
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }

@@ -31,7 +31,8 @@ int main(int argc, char** argv) {
     fclose(f);
     const int32_t w = (int32_t)m["w"], h = (int32_t)m["h"];
     const int64_t R = (int64_t)w * h * L;
-    const int64_t wn = (R + 63) / 64 * 64;
+    // room for the ROI plus the tile engine's 512 reserved nodes
+    const int64_t wn = (R + 512 + 63) / 64 * 64;
     std::vector<uint64_t> keys((size_t)wn);
     std::vector<uint32_t> queues((size_t)(4 * wn)), bitmaps((size_t)(3 * wn / 32));
     std::vector<float> dist((size_t)R);

@@ -287,6 +287,90 @@ def test_tile_engine_on_host_threads(driver, tmp_path, monkeypatch, X, Y, L, n, 
         assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
 
 
+@pytest.mark.parametrize("X,Y,L,n,seed,conc", [
+    (40, 34, 6, 16, 4, 1),     # class 6, workgroups one at a time (the first one does every visit)
+    (40, 34, 6, 16, 4, 4),     # concurrent workgroups taking tiles from the ring
+    (30, 26, 5, 12, 7, 3),     # generic class (5 layers)
+    (34, 30, 20, 6, 2, 3),     # class 20 (the 800 x 800 x 20 tiles)
+])
+def test_tile_engine_queue_form_on_host_threads(driver, tmp_path, monkeypatch, X, Y, L, n, seed, conc):
+    """The queue form of the tile engine (ORX_F_ASYNC: one persistent launch
+    per wave, tiles visited from a ring as they are queued, a state word per
+    tile): paths, fp32 costs, occupancy, history and counts equal the
+    oracle's, with workgroups one at a time or concurrent, and no
+    out-of-bounds access."""
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", str(conc))
+    nl = uneven(X, Y, L, n, seed=seed)
+    cfg = RouterConfig(iterations=2)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3, extra_meta={"flags": 1 | 32})
+    o = OraclePathFinder(nl, cfg)
+    for _ in range(2):
+        d = c.run_iteration(timeout=600)
+        od = o.run_iteration()
+        for k in ("routed", "failed", "edge_over", "node_over", "via_hops", "barrels", "settled", "relaxed",
+                  "path_hops"):
+            assert d[k] == od[k], k
+        for a, b in zip(c.paths(), o.paths()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(c.path_cost.view(np.uint32), o.path_cost.view(np.uint32))
+        assert np.array_equal(c.occ, o.occ)
+        assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
+
+
+def test_tile_queue_deadline_commits_nothing(driver, tmp_path, monkeypatch):
+    """The queue form's bounded wait: with a deadline far below a visit's
+    time (ORX_TILE_RUN_SECONDS), waiting workgroups give up and flag the
+    launch; every net of such a wave then fails with ORX_E_OVERFLOW and
+    commits nothing, the launch still ends, and nets of waves that finished
+    equal the oracle's."""
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "4")
+    monkeypatch.setenv("ORX_TILE_RUN_SECONDS", "0.000001")
+    nl = uneven(40, 34, 6, 16, seed=4)
+    cfg = RouterConfig(iterations=1)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3, extra_meta={"flags": 1 | 32})
+    o = OraclePathFinder(nl, cfg)
+    d = c.run_iteration(timeout=600)
+    o.run_iteration()
+    st = d["status"]
+    assert set(np.unique(st).tolist()) <= {0, -3}
+    assert (st == -3).sum() >= 1
+    failed = np.flatnonzero(st == -3)
+    assert (c.path_len[failed] == 0).all()
+    for k, (a, b) in enumerate(zip(c.paths(), o.paths())):
+        if st[k] == 0:
+            assert np.array_equal(a, b), k
+    # occupancy is exactly what the committed paths hold
+    occ = np.zeros_like(c.occ)
+    nb = nl.spec.neighbours()
+    for pth in c.paths():
+        for u, v in zip(pth[:-1], pth[1:]):
+            s = int(np.flatnonzero(nb[u] == v)[0])
+            occ[4 * u + s] += 1
+            occ[4 * v + (3 - s)] += 1
+    assert np.array_equal(occ, c.occ)
+
+
+@pytest.mark.parametrize("conc", [1, 4])
+def test_tile_queue_box_query_on_host_threads(driver, tmp_path, monkeypatch, conc):
+    """A whole-box query on the queue form (orx_sssp_box with ORX_F_TILES |
+    ORX_F_ASYNC) over a box of many tiles: dist bits and parents equal the
+    oracle's heap Dijkstra."""
+    from oracle.oracle import sssp_box as oracle_sssp_box
+    from orthoroute_amd.lattice import LatticeSpec
+    from tests.cpu_build.harness import sssp_box_host_threads
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", str(conc))
+    spec = LatticeSpec(70, 52, 6)
+    w = spec.random_edge_costs(5)
+    box = (2, 3, 66, 47)
+    src = int(spec.node(40, 20, 0))
+    dist, parent, stats = sssp_box_host_threads(spec, w, box, src, -1, tmp_path, grid=False, flags=16 | 32)
+    od, op, _, _, _ = oracle_sssp_box(spec, w, box, src)
+    assert np.array_equal(dist.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(parent, op)
+
+
 def test_tile_engine_portal_nets_on_host_threads(driver, tmp_path, monkeypatch):
     """Multi-terminal nets on the tile engine: targets are sinks (never
     expanded, also across a tile boundary), the best target by total."""

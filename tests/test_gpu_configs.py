@@ -46,28 +46,29 @@ def c1_query():
     return spec, base, w, src, dst, od, op
 
 
-@pytest.mark.parametrize("tiles", [True, False])
-def test_sssp_1000x1000x12_full_query_equals_oracle(device, c1_query, tiles):
+@pytest.mark.parametrize("tiles,queue", [(True, True), (True, False), (False, False)])
+def test_sssp_1000x1000x12_full_query_equals_oracle(device, c1_query, tiles, queue):
     spec, base, w, src, dst, od, op = c1_query
     lat = LatticeState(spec, base=base, device=device)
     lat.reprice(RouterConfig().params(0.5))
     assert np.array_equal(lat.wcost.cpu().numpy().view(np.uint32), w.view(np.uint32))
     ws = Workspace(device)
-    print(f"[c1] device query, tiles={tiles}", flush=True)
+    print(f"[c1] device query, tiles={tiles} queue={queue}", flush=True)
     dist, parent, rc = sssp_box(lat, (0, 0, spec.X, spec.Y), src, dst, full=True, delta=16.0, block=256, ws=ws,
-                                grid=not tiles, tiles=tiles)
+                                grid=not tiles, tiles=tiles, queue=queue)
     assert rc == 0
     assert np.array_equal(dist.cpu().numpy().view(np.uint32), od.view(np.uint32))
     assert np.array_equal(parent.cpu().numpy(), op)
 
 
-def test_uneven_800_iterations_1_2_equal_oracle(device):
+@pytest.mark.parametrize("queue", [False, True])
+def test_uneven_800_iterations_1_2_equal_oracle(device, queue):
     nl = uneven_800()
-    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384)
+    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384, tile_queue=queue)
     dev = DevicePathFinder(nl, cfg, device=device)
     orc = OraclePathFinder(nl, cfg)
     for it in range(2):
-        print(f"[c3] iteration {it + 1}", flush=True)
+        print(f"[c3] iteration {it + 1} queue={queue}", flush=True)
         sd = dev.run_iteration("batched")
         so = orc.run_iteration()
         for k in ("routed", "failed", "edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops",
@@ -86,10 +87,12 @@ def test_uneven_800_iterations_1_2_equal_oracle(device):
 SERVING_SEEDS = (168, 75, 117, 39)   # serving_job boards 256x256x16, 512x512x8, 128x128x32, 1024x1024x24
 
 
-def test_serving_stream_boards_served_equal_alone(device):
+@pytest.mark.parametrize("queue", [False, True])
+def test_serving_stream_boards_served_equal_alone(device, queue):
     boards = [serving_job(np.random.default_rng(s)) for s in SERVING_SEEDS]
     assert [(b.spec.X, b.spec.L) for b in boards] == [(256, 16), (512, 8), (128, 32), (1024, 24)]
-    cfg = RouterConfig(iterations=3, stop_at_zero_overuse=True, delta=16.0, block=1024, lds_nodes=16384)
+    cfg = RouterConfig(iterations=3, stop_at_zero_overuse=True, delta=16.0, block=1024, lds_nodes=16384,
+                       tile_queue=queue)
     max_it = 3
     want = []
     for nl in boards:

@@ -17,6 +17,11 @@ from orthoroute_amd.sssp import sssp_box
 
 pytestmark = pytest.mark.gpu
 
+# the search engines: tile engine in step form and in queue form (one
+# persistent launch per wave), and the one-workgroup-per-net engine
+ENGINES = {"tiles": dict(engine="tiles", tile_queue=False), "queue": dict(engine="tiles", tile_queue=True),
+           "wave": dict(engine="wave")}
+
 STAT_KEYS = ["routed", "failed", "edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops",
              "barrels", "settled", "relaxed", "path_hops"]
 
@@ -53,11 +58,11 @@ def _run_both(nl, cfg, iters, mode, device):
     return dev, orc
 
 
-@pytest.mark.parametrize("engine", ["tiles", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "queue", "wave"])
 @pytest.mark.parametrize("mode", ["batched", "plain"])
 def test_plain_64_ten_iterations_bit_exact(device, mode, engine):
     nl = plain_64()
-    cfg = RouterConfig(iterations=10, engine=engine)
+    cfg = RouterConfig(iterations=10, **ENGINES[engine])
     dev, orc = _run_both(nl, cfg, 10, mode, device)
     assert dev.history[-1].overuse == orc.history[-1]["edge_over"] + orc.history[-1]["node_over"]
 
@@ -79,15 +84,15 @@ def test_wide_bucket_large_block_bit_exact(device, lds_nodes):
     _run_both(nl, cfg, 3, "batched", device)
 
 
-@pytest.mark.parametrize("engine", ["tiles", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "queue", "wave"])
 @pytest.mark.parametrize("block", [256, 512, 1024])
 def test_block_sizes_bit_exact(device, block, engine):
     nl = uneven(64, 64, 6, 150, seed=5)
-    cfg = RouterConfig(iterations=3, block=block, engine=engine)
+    cfg = RouterConfig(iterations=3, block=block, **ENGINES[engine])
     _run_both(nl, cfg, 3, "batched", device)
 
 
-@pytest.mark.parametrize("engine", ["tiles", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "queue", "wave"])
 def test_largest_rois_bit_exact(device, engine):
     """The 16 largest nets of the 800x800x20 board (ROIs of 0.5-2.5 M nodes)
     against the oracle: on the tile engine, hundreds of tiles per net over
@@ -98,7 +103,7 @@ def test_largest_rois_bit_exact(device, engine):
     big = np.argsort(-nl.roi_nodes(), kind="stable")[:16]
     sub = nl.select(np.sort(big), "largest16")
     assert sub.roi_nodes().min() > 400_000
-    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384, engine=engine)
+    cfg = RouterConfig(iterations=2, delta=16.0, block=1024, lds_nodes=16384, **ENGINES[engine])
     _run_both(sub, cfg, 2, "batched", device)
 
 
