@@ -136,6 +136,10 @@ typedef struct orx_route_cfg {
     int32_t lds_nodes;      /* ROIs up to this many nodes keep keys in LDS   */
     int32_t flags;          /* ORX_F_* */
     int64_t* stats;         /* device [8] accumulators or NULL               */
+    int32_t max_groups;     /* tile engine: cap on the workgroups of a step  */
+                            /* or persistent launch (0: the whole chip); a   */
+                            /* server running several jobs at once gives     */
+                            /* each its share of the CUs                     */
 } orx_route_cfg;
 
 #define ORX_F_STATS 1       /* count settled nodes / relaxed arcs per query  */

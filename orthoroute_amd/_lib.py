@@ -77,7 +77,7 @@ class OrxWorkspace(C.Structure):
 class OrxRouteCfg(C.Structure):
     _fields_ = [
         ("delta", C.c_float), ("block", C.c_int32), ("lds_nodes", C.c_int32),
-        ("flags", C.c_int32), ("stats", P),
+        ("flags", C.c_int32), ("stats", P), ("max_groups", C.c_int32),
     ]
 
 

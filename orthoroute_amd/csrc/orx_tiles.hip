@@ -1148,10 +1148,12 @@ int tile_lds_ready(int32_t L) {
     return orx_host::fail(ORX_E_ARG, "no tile kernel for this layer count");
 }
 
-// Step workgroups per launch: as many as the LDS of the chip holds.
-int tile_grid(int32_t L) {
+// Step workgroups per launch: as many as the LDS of the chip holds, or the
+// caller's cap (max_groups > 0).
+int tile_grid(int32_t L, int32_t max_groups) {
     const int per_cu = std::max(1, (int)(kMaxDynLds / (size_t)tile_lds(L).bytes));
-    return device_cus() * per_cu;
+    const int g = device_cus() * per_cu;
+    return max_groups > 0 ? std::min(g, (int)max_groups) : g;
 }
 
 void launch_step(const TileStep& S, uint32_t k, int grid, size_t lds, hipStream_t s) {
@@ -1185,10 +1187,10 @@ uint64_t run_deadline_ticks() {
 
 // Workgroups of a persistent launch: the chip's, or fewer when the wave has
 // fewer tiles (every slot's tiles, bounded by its share of the extent).
-int run_grid(int32_t L, int32_t K, int64_t extent) {
+int run_grid(int32_t L, int32_t K, int64_t extent, int32_t max_groups) {
     const int64_t tile_nodes = (int64_t)kTX * tile_ty(L) * L;
     const int64_t tiles = (int64_t)K + 2 * ((extent + tile_nodes - 1) / tile_nodes);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(tile_grid(L), tiles));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(tile_grid(L, max_groups), tiles));
 }
 
 void launch_run(const TileStep& S, int grid, size_t lds, hipStream_t s) {
@@ -1213,7 +1215,7 @@ void launch_drain(const TileStep& S, uint32_t k0, size_t lds, hipStream_t s) {
 
 // One wave: fill, prep, `steps` step launches, drain, post.
 int launch_tile_wave(TileArgs A, const int32_t* d_wave_nets, const int64_t* d_ws_off, int32_t K, int64_t ws_extent,
-                     int32_t steps, int32_t block, hipStream_t s) {
+                     int32_t steps, int32_t block, int32_t max_groups, hipStream_t s) {
     if (K <= 0) return ORX_OK;
     A.S.slot_net = d_wave_nets;
     A.S.slot_off = d_ws_off;
@@ -1227,9 +1229,9 @@ int launch_tile_wave(TileArgs A, const int32_t* d_wave_nets, const int64_t* d_ws
     hipLaunchKernelGGL(tile_fill_kernel, dim3(fill_grid), dim3(256), 0, s, A.S, A.extent);
     hipLaunchKernelGGL(tile_prep_kernel<256>, dim3(K), dim3(256), 0, s, A, 1);
     if (A.S.flags & ORX_F_ASYNC) {
-        launch_run(A.S, run_grid(A.S.L, K, A.extent), lds, s);
+        launch_run(A.S, run_grid(A.S.L, K, A.extent, max_groups), lds, s);
     } else {
-        const int grid = tile_grid(A.S.L);
+        const int grid = tile_grid(A.S.L, max_groups);
         for (int32_t k = 0; k < steps; ++k) launch_step(A.S, (uint32_t)k, grid, lds, s);
         launch_drain(A.S, (uint32_t)steps, lds, s);
     }
@@ -1295,7 +1297,7 @@ int sssp_box_tiles(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, in
     if (e == hipSuccess) e = hipMallocAsync((void**)&d_acc, 2 * sizeof(unsigned long long), ps);
     if (e == hipSuccess) e = hipMemsetAsync(d_acc, 0, 2 * sizeof(unsigned long long), ps);
     const size_t lds = (size_t)tile_lds(lat->L).bytes;
-    const int grid = tile_grid(lat->L);
+    const int grid = tile_grid(lat->L, cfg->max_groups);
     if (e == hipSuccess) {
         const int fill_grid = (int)std::max<int64_t>(1, std::min<int64_t>(device_cus() * 8, (R / 2 + 255) / 256));
         hipLaunchKernelGGL(tile_fill_kernel, dim3(fill_grid), dim3(256), 0, ps, A.S, R);
@@ -1306,7 +1308,7 @@ int sssp_box_tiles(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t w, in
     const bool async = (cfg->flags & ORX_F_ASYNC) != 0;
     if (async && e == hipSuccess) {
         // queue form: one persistent launch runs the search to its end
-        launch_run(A.S, run_grid(lat->L, 1, R), lds, ps);
+        launch_run(A.S, run_grid(lat->L, 1, R, cfg->max_groups), lds, ps);
         e = hipGetLastError();
     }
     if (!async && e == hipSuccess) e = hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal);
@@ -1408,7 +1410,8 @@ int orx_route_tiles(const orx_lattice* lat, const orx_params* p, const orx_nets*
     const hipStream_t s = (hipStream_t)stream;
     for (int32_t i = 0; i < n_waves; ++i) {
         const int32_t a = h_wave_ptr[i], K = h_wave_ptr[i + 1] - a;
-        rc = launch_tile_wave(A, d_nets + a, d_ws_off + a, K, h_wave_ws[i], h_wave_steps[i], cfg->block, s);
+        rc = launch_tile_wave(A, d_nets + a, d_ws_off + a, K, h_wave_ws[i], h_wave_steps[i], cfg->block,
+                              cfg->max_groups, s);
         if (rc) return rc;
     }
     return ORX_OK;

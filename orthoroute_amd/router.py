@@ -65,6 +65,7 @@ class RouterConfig:
     delta_max: float = 1048576.0
     engine: str = "tiles"             # "tiles": tile-step search (orx_route_tiles); "wave": one workgroup per net
     tile_queue: bool = False          # tile engine, queue form: one persistent launch per wave (ORX_F_ASYNC)
+    small_wave_nodes: int = 16384     # tile engine: waves whose largest ROI is at most this run one workgroup per net
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -448,6 +449,11 @@ class DevicePathFinder:
         self.astar = self.cfg.astar_bounds(self.spec.base_cost_minima() if base is None else base)
         self.iteration = 0
         self.max_groups = 0
+        # jobs sharing the GPU with this one (the serving loop sets it): the
+        # tile engine then launches at most its share of the CUs' workgroups,
+        # so the jobs' kernels run side by side instead of in turn
+        self.concurrent_jobs = 1
+        self._cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.delta_mult = 1.0     # adaptive widening of cfg.delta (adapt_delta)
         self.history: list[IterationStats] = []
         self._pending: _Pending | None = None
@@ -499,7 +505,8 @@ class DevicePathFinder:
         c = self.cfg
         fl = flags | (_lib.ORX_F_STATS if c.collect_stats else 0) | \
             (_lib.ORX_F_ASYNC if c.engine == "tiles" and c.tile_queue else 0)
-        return _lib.OrxRouteCfg(self.delta, int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr())
+        share = 0 if self.concurrent_jobs <= 1 else max(16, self._cus // int(self.concurrent_jobs))
+        return _lib.OrxRouteCfg(self.delta, int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr(), share)
 
     def _adapt_delta(self, st: "IterationStats") -> None:
         """Widen delta x4 (up to delta_max) after an iteration whose searches
@@ -567,6 +574,49 @@ class DevicePathFinder:
         per = self._tile_steps[flat_nets]
         return np.maximum.reduceat(per, wave_ptr[:-1]).astype(np.int32)
 
+    def _route_calls(self, params, rcfg, flat_nets, wave_ptr, wave_max, wave_ws, d_nets, d_off) -> list:
+        """The entry-point calls that route an iteration's waves, in order:
+        orx_route_tiles for the tile engine, orx_route_waves for one
+        workgroup per net.  On the tile engine, each run of consecutive waves
+        whose largest ROI is at most cfg.small_wave_nodes goes to
+        orx_route_waves (a launch or two per wave instead of a tile step
+        sequence; every engine reaches the same fixpoint, so the result is
+        the same)."""
+        c = self.cfg
+        if c.engine not in ("tiles", "wave"):
+            raise ValueError("engine must be 'tiles' or 'wave'")
+        nw = len(wave_ptr) - 1
+        if nw <= 0:
+            return []
+        if c.engine == "wave":
+            small = np.ones(nw, bool)
+        else:
+            small = wave_max <= int(c.small_wave_nodes) if c.small_wave_nodes > 0 else np.zeros(nw, bool)
+        wave_steps = self._wave_steps(flat_nets, wave_ptr) if not small.all() else None
+        lat, pr, nets, ws, rc = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
+                                 C.byref(self.ws.struct), C.byref(rcfg))
+        calls = []
+        i = 0
+        while i < nw:
+            j = i + 1
+            while j < nw and small[j] == small[i]:
+                j += 1
+            a = int(wave_ptr[i])
+            ptr = np.ascontiguousarray(wave_ptr[i:j + 1] - a, np.int32)
+            wws = np.ascontiguousarray(wave_ws[i:j], np.int64)
+            dn, do = d_nets.data_ptr() + 4 * a, d_off.data_ptr() + 8 * a
+            if small[i]:
+                wmax = np.ascontiguousarray(wave_max[i:j], np.int64)
+                calls.append(("orx_route_waves", (lat, pr, nets, dn, do, j - i, ptr.ctypes.data, wmax.ctypes.data,
+                                                  wws.ctypes.data, ws, rc), (ptr, wmax, wws)))
+            else:
+                wst = np.ascontiguousarray(wave_steps[i:j], np.int32)
+                calls.append(("orx_route_tiles", (lat, pr, nets, dn, do, j - i, ptr.ctypes.data, wws.ctypes.data,
+                                                  wst.ctypes.data, ws, rc), (ptr, wws, wst)))
+            i = j
+        self._call_keep = calls      # host arrays stay alive while the calls run
+        return [(e, a) for e, a, _ in calls]
+
     def ws_cap(self) -> int:
         """Workspace nodes one launch may use (at least the largest ROI)."""
         r64 = (int(self.nets.roi_nodes.max(initial=1)) + 63) // 64 * 64
@@ -622,25 +672,14 @@ class DevicePathFinder:
             rcfg = self.route_cfg()
             ev[1].record(stream)
             exec_graph = None
-            if self.cfg.engine == "tiles":
-                entry = "orx_route_tiles"
-                wave_steps = self._wave_steps(flat_nets, wave_ptr)
-                route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
-                              d_nets.data_ptr(), d_off.data_ptr(), len(waves), wave_ptr.ctypes.data,
-                              wave_ws.ctypes.data, wave_steps.ctypes.data, C.byref(self.ws.struct), C.byref(rcfg))
-            elif self.cfg.engine == "wave":
-                entry = "orx_route_waves"
-                route_args = (C.byref(self.lat.struct), C.byref(params), C.byref(self.nets.struct),
-                              d_nets.data_ptr(), d_off.data_ptr(), len(waves), wave_ptr.ctypes.data,
-                              wave_max.ctypes.data, wave_ws.ctypes.data, C.byref(self.ws.struct), C.byref(rcfg))
-            else:
-                raise ValueError("engine must be 'tiles' or 'wave'")
+            calls = self._route_calls(params, rcfg, flat_nets, wave_ptr, wave_max, wave_ws, d_nets, d_off)
             if graph and len(waves):
                 cs = C.c_void_p()
                 _lib.call("orx_graph_begin", C.byref(cs))
                 g = C.c_void_p()
                 try:
-                    _lib.call(entry, *route_args, cs)
+                    for entry, route_args in calls:
+                        _lib.call(entry, *route_args, cs)
                 except Exception:
                     # end the capture (which frees the stream) and drop the graph
                     if _lib.load().orx_graph_end(cs, C.byref(g)) == _lib.ORX_OK:
@@ -654,7 +693,8 @@ class DevicePathFinder:
                     _lib.load().orx_graph_destroy(exec_graph)
                     raise
             else:
-                _lib.call(entry, *route_args, s)
+                for entry, route_args in calls:
+                    _lib.call(entry, *route_args, s)
             ev[2].record(stream)
             _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
                       self._iter_out.data_ptr(), s)

@@ -156,7 +156,7 @@ class RoutingServer:
             self.peak_in_use = max(self.peak_in_use, self.in_use)
             self.active[job.job_id] = (job, runner, slot)
             self.peak_active = max(self.peak_active, len(self.active))
-            runner.launch_iteration()
+            self._launch(runner)
             admitted = True
         return admitted
 
@@ -188,8 +188,16 @@ class RoutingServer:
                 self._retire(job, runner, slot)
                 del runner
             else:
-                runner.launch_iteration()
+                self._launch(runner)
         return progressed
+
+    def _launch(self, runner) -> None:
+        """Enqueue the runner's next iteration, telling it how many jobs
+        share the device now (a runner with ``concurrent_jobs`` sizes its
+        launches to its share; results do not depend on it)."""
+        if hasattr(runner, "concurrent_jobs"):
+            runner.concurrent_jobs = max(1, len(self.active))
+        runner.launch_iteration()
 
     # -- the loop ------------------------------------------------------------
     def run(self, jobs: list[RouteJob], time_limit: float | None = None) -> dict:

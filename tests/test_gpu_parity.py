@@ -17,9 +17,13 @@ from orthoroute_amd.sssp import sssp_box
 
 pytestmark = pytest.mark.gpu
 
-# the search engines: tile engine in step form and in queue form (one
-# persistent launch per wave), and the one-workgroup-per-net engine
-ENGINES = {"tiles": dict(engine="tiles", tile_queue=False), "queue": dict(engine="tiles", tile_queue=True),
+# the search engines: the tile engine on every wave, in step form and in
+# queue form (one persistent launch per wave); the default hybrid (waves of
+# small ROIs one workgroup per net, the rest on the tile engine); and one
+# workgroup per net throughout
+ENGINES = {"tiles": dict(engine="tiles", tile_queue=False, small_wave_nodes=0),
+           "queue": dict(engine="tiles", tile_queue=True, small_wave_nodes=0),
+           "hybrid": dict(engine="tiles", small_wave_nodes=4096),
            "wave": dict(engine="wave")}
 
 STAT_KEYS = ["routed", "failed", "edge_over", "edge_over_count", "node_over", "node_over_count", "via_hops",
@@ -58,7 +62,7 @@ def _run_both(nl, cfg, iters, mode, device):
     return dev, orc
 
 
-@pytest.mark.parametrize("engine", ["tiles", "queue", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "queue", "hybrid", "wave"])
 @pytest.mark.parametrize("mode", ["batched", "plain"])
 def test_plain_64_ten_iterations_bit_exact(device, mode, engine):
     nl = plain_64()
@@ -84,7 +88,7 @@ def test_wide_bucket_large_block_bit_exact(device, lds_nodes):
     _run_both(nl, cfg, 3, "batched", device)
 
 
-@pytest.mark.parametrize("engine", ["tiles", "queue", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "queue", "hybrid", "wave"])
 @pytest.mark.parametrize("block", [256, 512, 1024])
 def test_block_sizes_bit_exact(device, block, engine):
     nl = uneven(64, 64, 6, 150, seed=5)
@@ -92,7 +96,7 @@ def test_block_sizes_bit_exact(device, block, engine):
     _run_both(nl, cfg, 3, "batched", device)
 
 
-@pytest.mark.parametrize("engine", ["tiles", "queue", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "queue", "hybrid", "wave"])
 def test_largest_rois_bit_exact(device, engine):
     """The 16 largest nets of the 800x800x20 board (ROIs of 0.5-2.5 M nodes)
     against the oracle: on the tile engine, hundreds of tiles per net over
