@@ -66,6 +66,7 @@ class RouterConfig:
     engine: str = "tiles"             # "tiles": tile-step search (orx_route_tiles); "wave": one workgroup per net
     tile_queue: bool = False          # tile engine, queue form: one persistent launch per wave (ORX_F_ASYNC)
     small_wave_nodes: int = 16384     # tile engine: waves whose largest ROI is at most this run one workgroup per net
+    tile_groups: int = 0              # tile engine: cap on the workgroups of a launch (0: the whole chip)
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -449,11 +450,12 @@ class DevicePathFinder:
         self.astar = self.cfg.astar_bounds(self.spec.base_cost_minima() if base is None else base)
         self.iteration = 0
         self.max_groups = 0
-        # jobs sharing the GPU with this one (the serving loop sets it): the
-        # tile engine then launches at most its share of the CUs' workgroups,
-        # so the jobs' kernels run side by side instead of in turn
+        # jobs sharing the GPU with this one (the serving loop sets it).  A
+        # tile-step workgroup holds a whole CU's LDS, and such launches from
+        # one job stall the dispatch of the others' kernels; so while jobs
+        # share the device, every wave runs one workgroup per net (the same
+        # fixpoint, so the same result)
         self.concurrent_jobs = 1
-        self._cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.delta_mult = 1.0     # adaptive widening of cfg.delta (adapt_delta)
         self.history: list[IterationStats] = []
         self._pending: _Pending | None = None
@@ -505,8 +507,8 @@ class DevicePathFinder:
         c = self.cfg
         fl = flags | (_lib.ORX_F_STATS if c.collect_stats else 0) | \
             (_lib.ORX_F_ASYNC if c.engine == "tiles" and c.tile_queue else 0)
-        share = 0 if self.concurrent_jobs <= 1 else max(16, self._cus // int(self.concurrent_jobs))
-        return _lib.OrxRouteCfg(self.delta, int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr(), share)
+        return _lib.OrxRouteCfg(self.delta, int(c.block), int(c.lds_nodes), fl, self._route_stats.data_ptr(),
+                                int(c.tile_groups))
 
     def _adapt_delta(self, st: "IterationStats") -> None:
         """Widen delta x4 (up to delta_max) after an iteration whose searches
@@ -588,7 +590,7 @@ class DevicePathFinder:
         nw = len(wave_ptr) - 1
         if nw <= 0:
             return []
-        if c.engine == "wave":
+        if c.engine == "wave" or self.concurrent_jobs > 1:
             small = np.ones(nw, bool)
         else:
             small = wave_max <= int(c.small_wave_nodes) if c.small_wave_nodes > 0 else np.zeros(nw, bool)

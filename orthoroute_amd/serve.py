@@ -193,8 +193,9 @@ class RoutingServer:
 
     def _launch(self, runner) -> None:
         """Enqueue the runner's next iteration, telling it how many jobs
-        share the device now (a runner with ``concurrent_jobs`` sizes its
-        launches to its share; results do not depend on it)."""
+        share the device now (a runner with ``concurrent_jobs`` picks launch
+        shapes that leave room for the others; results do not depend on
+        it)."""
         if hasattr(runner, "concurrent_jobs"):
             runner.concurrent_jobs = max(1, len(self.active))
         runner.launch_iteration()

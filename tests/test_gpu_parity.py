@@ -22,6 +22,7 @@ pytestmark = pytest.mark.gpu
 # small ROIs one workgroup per net, the rest on the tile engine); and one
 # workgroup per net throughout
 ENGINES = {"tiles": dict(engine="tiles", tile_queue=False, small_wave_nodes=0),
+           "tiles16": dict(engine="tiles", tile_queue=False, small_wave_nodes=0, tile_groups=16),
            "queue": dict(engine="tiles", tile_queue=True, small_wave_nodes=0),
            "hybrid": dict(engine="tiles", small_wave_nodes=4096),
            "wave": dict(engine="wave")}
@@ -88,7 +89,7 @@ def test_wide_bucket_large_block_bit_exact(device, lds_nodes):
     _run_both(nl, cfg, 3, "batched", device)
 
 
-@pytest.mark.parametrize("engine", ["tiles", "queue", "hybrid", "wave"])
+@pytest.mark.parametrize("engine", ["tiles", "tiles16", "queue", "hybrid", "wave"])
 @pytest.mark.parametrize("block", [256, 512, 1024])
 def test_block_sizes_bit_exact(device, block, engine):
     nl = uneven(64, 64, 6, 150, seed=5)
