@@ -16,10 +16,10 @@ Printed: one JSON line.
 - ``roofline`` of the route launches: algorithmic bytes of the search
   (12 B per relaxed arc + 16 B per settled node, the label-setting count)
   over the route time, measured with HIP events on the stream the waves run
-  on.  With the tile engine one launch is one orx_route_tiles call (one
-  iteration's prep / step / drain / post kernels); ``traffic`` is the
-  calibrated FETCH_SIZE + WRITE_SIZE of the same launch from the committed
-  rocprofv3 summary.
+  on.  With the tile engine one launch is one iteration's route kernels
+  (orx_route_tiles for waves of large ROIs, orx_route_waves for the waves of
+  small ones); ``traffic`` is the calibrated FETCH_SIZE + WRITE_SIZE of the
+  same launch from the committed rocprofv3 summary.
 - ``graph``: the timed iterations repeated from the same state with each
   iteration's launches captured into one hipGraph (capture included).
 - ``cpu_baseline``: the oracle on one host core, on iteration 1 of the same
@@ -59,9 +59,11 @@ if _queues < 8:
 
 HBM_PEAK_GBS = 8000.0
 SSSP_TRAFFIC_FILE = REPO / "profiles" / "sssp_tiles_traffic_1000x1000x12.json"
-TRAFFIC_FILES = {"tiles": REPO / "profiles" / "route_tiles_traffic_uneven_800x800x20.json",
+TRAFFIC_FILES = {"tiles": REPO / "profiles" / "route_traffic_uneven_800x800x20.json",
                  "wave": REPO / "profiles" / "route_wave_traffic_uneven_800x800x20.json"}
-ROUTE_KERNELS = {"tiles": "orx_route_tiles (tile_prep, tile_step, tile_drain, tile_post kernels of one iteration)",
+ROUTE_KERNELS = {"tiles": "the route kernels of one iteration: orx_route_tiles (tile_fill, tile_prep, tile_step, "
+                          "tile_drain, tile_post) for waves whose largest ROI exceeds 16384 nodes, orx_route_waves "
+                          "(ws_fill, route_wave_kernel) for the others",
                  "wave": "route_wave_kernel"}
 
 
@@ -387,9 +389,9 @@ def roi_bench(n: int = 32, samples: int = 4, with_cpu: bool = True) -> dict:
 
 
 def _traffic(engine: str):
-    """HBM bytes per route launch (the tile engine: one orx_route_tiles call,
-    i.e. one iteration's route kernels; the wave engine: one route_wave_kernel
-    dispatch) from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    """HBM bytes per route launch (the tile engine: one iteration's route
+    kernels; the wave engine: one route_wave_kernel dispatch) from the
+    committed rocprofv3 PMC summary (FETCH_SIZE +
     WRITE_SIZE, calibrated by a copy of known size), or None."""
     try:
         t = json.loads(TRAFFIC_FILES[engine].read_text())
@@ -623,8 +625,8 @@ def main() -> int:
         cpu_all = cpu_baseline_all_cores(nl, cfg)
 
     traffic, traffic_src = _traffic(args.engine)
-    # one launch: one orx_route_tiles call (an iteration) for the tile engine,
-    # one route_wave_kernel dispatch (a wave) for the wave engine
+    # one launch: one iteration's route kernels for the tile engine, one
+    # route_wave_kernel dispatch (a wave) for the wave engine
     n_dispatch = args.steps if args.engine == "tiles" else waves
     out = {
         "metric": "nets routed/s (batched disjoint-ROI PathFinder iteration)",
