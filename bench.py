@@ -503,7 +503,7 @@ def main() -> int:
     block = args.block if args.block is not None else tuned["block"]
     lds_nodes = args.lds_nodes if args.lds_nodes is not None else tuned["lds_nodes"]
     cfg = RouterConfig(iterations=args.warmup + args.steps, delta=delta, block=block, lds_nodes=lds_nodes,
-                       graph=args.graph, engine=args.engine)
+                       graph=args.graph, engine=args.engine, small_wave_nodes=int(tuned["small_wave_nodes"]))
 
     _log(f"{nl.name}: {args.warmup} warm-up + {args.steps} timed iterations (delta {delta}, block {block})")
     dev = DevicePathFinder(nl, cfg)

@@ -124,6 +124,25 @@ def roi_boxes(spec: LatticeSpec, src: np.ndarray, dst: np.ndarray, margin: int) 
     return np.stack([x0, y0, x1 - x0 + 1, y1 - y0 + 1], axis=1).astype(np.int32)
 
 
+def long_nets_full_board(nl: Netlist, steps: int = 125) -> Netlist:
+    """The long-net half of the reference engine's ROI rule: a net whose
+    pins are at least ``steps`` apart (|dx| + |dy| in cells, UP:7980-7982)
+    routes on the full graph (UP:8219-8262), here the whole board as its
+    box; shorter nets keep their box.  Multi-terminal nets keep theirs."""
+    sx, sy, _ = nl.spec.coords(nl.src)
+    tx, ty, _ = nl.spec.coords(nl.dst)
+    d = np.abs(tx.astype(np.int64) - sx) + np.abs(ty.astype(np.int64) - sy)
+    box = nl.box.copy()
+    long = d >= steps
+    if nl.terms is not None:
+        ns, _ = nl.terms.counts()
+        long &= ns == 0
+    box[long] = (0, 0, nl.spec.X, nl.spec.Y)
+    meta = dict(nl.meta, full_board_steps=steps, full_board_nets=int(long.sum()))
+    return Netlist(nl.spec, nl.src.copy(), nl.dst.copy(), box, f"{nl.name}_full{steps}", meta,
+                   nl.terms)
+
+
 def _place_two_pin(rng, spec: LatticeSpec, lengths: np.ndarray, used: np.ndarray,
                    forbid: np.ndarray | None = None, max_tries: int = 64):
     """For each net: a random Manhattan split (|dx|, |dy|) of its length and

@@ -272,3 +272,15 @@ def test_hotset_mode_bit_exact(device):
     cfg = RouterConfig(iterations=12, hotset=True, delta=16.0, block=1024, lds_nodes=16384)
     dev, orc = _run_both(nl, cfg, 12, "batched", device)
     assert min(s.routed for s in dev.history[1:]) < nl.n
+
+
+def test_long_nets_on_the_full_board_bit_exact(device):
+    """The reference's long-net rule (nets of at least 60 steps here route on
+    the whole board): 26 of 120 nets search the full 160x120x6 lattice on
+    the tile engine, beside boxed short nets; 3 iterations equal the
+    oracle's sequential run."""
+    from orthoroute_amd.netgen import long_nets_full_board
+    nl = long_nets_full_board(uneven(160, 120, 6, 120, seed=21), 60)
+    assert nl.meta["full_board_nets"] == 26
+    cfg = RouterConfig(iterations=3, delta=16.0, block=1024, lds_nodes=16384)
+    _run_both(nl, cfg, 3, "batched", device)

@@ -228,3 +228,19 @@ def test_lattice_arc_counts(X, Y, L):
     assert int(via.sum()) == 2 * X * Y * (L - 1)
     if (X, Y, L) == (64, 64, 6):
         assert len(ix) == 89_344
+
+
+def test_long_nets_get_the_whole_board():
+    """long_nets_full_board: |dx| + |dy| >= steps -> box (0, 0, X, Y); shorter
+    nets keep their bounding box + margin; the threshold is inclusive."""
+    import numpy as np
+    from orthoroute_amd.lattice import LatticeSpec
+    from orthoroute_amd.netgen import Netlist, long_nets_full_board, roi_boxes
+    spec = LatticeSpec(50, 40, 4)
+    src = np.array([spec.node(1, 1, 0), spec.node(5, 5, 0), spec.node(10, 3, 0)], np.int32)
+    dst = np.array([spec.node(21, 1, 0), spec.node(5, 24, 0), spec.node(12, 4, 0)], np.int32)
+    nl = Netlist(spec, src, dst, roi_boxes(spec, src, dst, 4), "t")
+    out = long_nets_full_board(nl, 20)          # lengths 20, 19, 3
+    assert out.box[0].tolist() == [0, 0, 50, 40]
+    assert out.box[1].tolist() == nl.box[1].tolist() and out.box[2].tolist() == nl.box[2].tolist()
+    assert out.meta["full_board_nets"] == 1
