@@ -29,6 +29,9 @@ Printed: one JSON line.
   CPU baseline and must give the same dist bits and parents.
 - ``backplane_plain_vs_batched``: the 400x400x16 backplane, plain one net per
   launch against batched waves, iterations 1-2 in lockstep.
+- ``seam``: the drop-in seam's find_path_fullgraph_gpu_seeds and
+  find_path_roi_gpu on the CSR of a 400x400x16 lattice, with the oracle's
+  heap search as CPU baseline and path check.
 - ``serving``: a Poisson stream of routing jobs through the
   continuous-batching server, with throughput and p50/p99 job latency.
 
@@ -226,6 +229,101 @@ def sssp_microbench(n_queries: int, delta: float, block: int, seed: int = 1, til
         out["same_as_oracle"] = bool(np.array_equal(od.view(np.uint32), first[0].view(np.uint32)) and
                                      np.array_equal(op, first[1]))
         out["speedup_vs_cpu"] = dt * 1e3 / out["ms_per_query"]
+    return out
+
+
+def seam_bench(n_queries: int = 8, seed: int = 11, with_cpu: bool = True) -> dict:
+    """The drop-in seam (HipDijkstra) on the CSR of a 400x400x16 lattice
+    (2.56 M nodes, 9.9 M arcs; U[1,4) costs, 5% x10, node penalties in
+    [0, 0.5)), costs and penalties resident on the device:
+    - find_path_fullgraph_gpu_seeds: 6 priced seeds within 40 columns of the
+      left edge, 6 priced targets within 40 of the right edge, the whole-chip
+      form and the parallel parent walk;
+    - find_path_roi_gpu: a 150x150-cell box ROI (360 K nodes) between two
+      corners, src and dst inside.
+    Each query is synchronous (it returns the host path).  The oracle's heap
+    search on the first full-graph query (one core) gives the CPU baseline,
+    the label-setting counts for the algorithmic bytes (12 B per relaxed arc
+    + 4 B of its penalty + 16 B per settled node), and the path to compare."""
+    import numpy as np
+    import torch
+    from orthoroute_amd.lattice import LatticeSpec, in_rank_and_transpose
+    from orthoroute_amd.solver import HipDijkstra
+
+    spec = LatticeSpec(400, 400, 16)
+    indptr, indices, _, _ = spec.to_csr()
+    rng = np.random.default_rng(seed)
+    E = len(indices)
+    costs = rng.uniform(1.0, 4.0, E).astype(np.float32)
+    costs[rng.random(E) < 0.05] *= np.float32(10.0)
+    pen = rng.uniform(0.0, 0.5, spec.N).astype(np.float32)
+    hip = HipDijkstra(indptr=indptr, indices=indices)
+    d_costs = torch.from_numpy(costs).cuda()
+    d_pen = torch.from_numpy(pen).cuda()
+
+    def side(lo, hi, k):
+        return np.array([spec.node(int(x), int(y), int(z)) for x, y, z in
+                         zip(rng.integers(lo, hi, k), rng.integers(0, spec.Y, k), rng.integers(0, spec.L, k))],
+                        np.int32)
+
+    queries = [(side(5, 40, 6), side(360, 395, 6), rng.uniform(0, 2, 6).astype(np.float32),
+                rng.uniform(0, 2, 6).astype(np.float32)) for _ in range(n_queries + 1)]
+
+    def full(q):
+        s, tg, sc, tc = q
+        return hip.find_path_fullgraph_gpu_seeds(costs=d_costs, src_seeds=s, dst_targets=tg, src_seed_costs=sc,
+                                                 dst_target_costs=tc, node_penalty=d_pen)
+
+    full(queries[0])                      # warm-up (buffers, graph set-up)
+    torch.cuda.synchronize()
+    times, paths = [], []
+    for q in queries[1:]:
+        t0 = time.perf_counter()
+        paths.append(full(q))
+        times.append(time.perf_counter() - t0)
+    out = {"workload": "lattice_csr_400x400x16", "nodes": spec.N, "arcs": E,
+           "fullgraph_seeds": {"queries": n_queries, "ms_per_query": float(np.mean(times)) * 1e3,
+                               "p50_ms": float(np.median(times)) * 1e3,
+                               "mean_path_nodes": float(np.mean([len(p) for p in paths if p]))}}
+    # ROI form: a 150 x 150 box (all layers) between two corners
+    x0, y0, w = 120, 130, 150
+    xs, ys, zs = np.meshgrid(np.arange(x0, x0 + w), np.arange(y0, y0 + w), np.arange(spec.L), indexing="ij")
+    roi_nodes = np.sort(spec.node(xs.ravel(), ys.ravel(), zs.ravel()).astype(np.int32))
+    g2r = np.full(spec.N, -1, np.int32)
+    g2r[roi_nodes] = np.arange(len(roi_nodes), dtype=np.int32)
+    d_roi, d_g2r = torch.from_numpy(roi_nodes).cuda(), torch.from_numpy(g2r).cuda()
+    d_roi_pen = torch.from_numpy(pen[roi_nodes]).cuda()
+    src, dst = int(spec.node(x0 + 2, y0 + 3, 0)), int(spec.node(x0 + w - 3, y0 + w - 2, 0))
+    hip.find_path_roi_gpu(src, dst, d_costs, d_roi, d_g2r, node_penalty=d_roi_pen)
+    torch.cuda.synchronize()
+    rt = []
+    for _ in range(n_queries):
+        t0 = time.perf_counter()
+        rp = hip.find_path_roi_gpu(src, dst, d_costs, d_roi, d_g2r, node_penalty=d_roi_pen)
+        rt.append(time.perf_counter() - t0)
+    out["roi"] = {"roi_nodes": int(len(roi_nodes)), "queries": n_queries,
+                  "ms_per_query": float(np.mean(rt)) * 1e3, "path_nodes": len(rp) if rp else 0}
+    if with_cpu:
+        from oracle.oracle import csr_path, sssp_csr
+        rank, in_indptr, in_src = in_rank_and_transpose(spec.N, indptr, indices)
+        s, tg, sc, tc = queries[1]
+        t0 = time.perf_counter()
+        rc, best, _, parent, (settled, relaxed) = sssp_csr(spec.N, indptr, indices, costs, rank, in_indptr, in_src,
+                                                            pen=pen, seeds=s, seed_cost=sc, targets=tg, tg_cost=tc,
+                                                            counts=True)
+        dt = time.perf_counter() - t0
+        alg = 16 * relaxed + 16 * settled
+        dev_s = times[0]
+        out["fullgraph_seeds"].update({
+            "same_as_oracle": csr_path(best, parent) == paths[0],
+            "algorithmic_bytes_first_query": alg, "settled_first_query": settled, "relaxed_first_query": relaxed,
+            "gteps_first_query": relaxed / dev_s / 1e9,
+            "roofline_first_query": {"bound": "hbm", "achieved": alg / dev_s / 1e9, "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": alg / dev_s / 1e9 / HBM_PEAK_GBS,
+                                     "timing": "host clock around the synchronous query (search, walk, path copy)"},
+            "cpu_baseline": {"value": dt * 1e3, "unit": "ms/query", "cores": 1, "kind": "port",
+                             "sample": f"the first timed query, oracle heap search on one core, {dt:.2f} s on "
+                                       f"{_cpu_model()}"}})
     return out
 
 
@@ -462,6 +560,7 @@ def main() -> int:
     ap.add_argument("--no-graph-leg", action="store_true",
                     help="skip the timed iterations repeated with hipGraph capture")
     ap.add_argument("--no-backplane", action="store_true")
+    ap.add_argument("--no-seam", action="store_true")
     ap.add_argument("--backplane-iterations", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-count", action="store_true",
@@ -587,6 +686,14 @@ def main() -> int:
                                     args.backplane_iterations)
         torch.cuda.empty_cache()
 
+    seam = None
+    if not args.no_seam:
+        _log("seam: find_path_fullgraph_gpu_seeds / find_path_roi_gpu on a 400x400x16 lattice CSR")
+        seam = seam_bench(with_cpu=not args.no_cpu)
+        _log(f"seam: {seam['fullgraph_seeds']['ms_per_query']:.2f} ms per full-graph query, "
+             f"{seam['roi']['ms_per_query']:.2f} ms per ROI query")
+        torch.cuda.empty_cache()
+
     serving = None
     if not args.no_serve:
         _log(f"serving: {args.serve_jobs} jobs at {args.serve_rate} jobs/s")
@@ -666,6 +773,7 @@ def main() -> int:
         "cpu_baseline_all_cores": cpu_all,
         "sssp_microbench": sssp,
         "serving": serving,
+        "seam": seam,
         "portal_nets": portal,
         "convergence": converge,
         "roi_bfs": roi,

@@ -328,7 +328,14 @@ int orx_sssp_box_noalloc(const orx_lattice* lat, int32_t x0, int32_t y0, int32_t
  * ROI index (NULL = none); allowed bitmap over global ids (NULL = all).
  * Enqueues the search; d_key[R] receives the packed dist/hops/slot keys and
  * d_best[0] the chosen target (global id, -1 none).  d_scratch needs
- * 4*R + 4*ceil(R/32) words. */
+ * 4*R + 4*ceil(R/32) words.
+ * cfg->flags ORX_F_GRID: the whole-chip form (the persistent full-graph
+ * kernel of PK:260-516, driven from CD:5516-5850): grid-wide near-far step
+ * launches as a device-side state machine, replayed from a captured graph
+ * until done, with the same keys and chosen target as the one-workgroup
+ * form; it synchronises with the caller's stream before it starts and
+ * returns when the result is complete; d_scratch then needs
+ * 4*R + 4*ceil(R/32) + 64 words. */
 typedef struct orx_csr {
     int64_t N, E;
     const int32_t* indptr;
@@ -373,6 +380,17 @@ int orx_roi_release(const int32_t* d_roi, const int64_t* d_out, int64_t cap, int
 int orx_backtrace_csr(const orx_csr* g, const uint64_t* d_key, const int32_t* d_g2r,
                       const int32_t* d_target, int32_t* d_path, int32_t cap, int32_t* d_len,
                       orx_stream_t stream);
+
+/* The same walk on the whole chip (backtrace_parent CD:5857-5900 walks one
+ * lane per net): the parent of every ROI node from its key slot, 32-hop
+ * jumps by pointer doubling, then the segments between nodes 32 hops apart
+ * filled in parallel; a path of h hops costs ~h/32 + 32 dependent loads.
+ * roi_nodes / g2r as for orx_sssp_csr (NULL = whole graph, R == N);
+ * d_scratch needs 3*R words (the search scratch may be reused).  Same
+ * outputs as orx_backtrace_csr. */
+int orx_backtrace_csr_roi(const orx_csr* g, const uint64_t* d_key, int64_t R, const int32_t* d_roi_nodes,
+                          const int32_t* d_g2r, const int32_t* d_target, int32_t* d_path, int32_t cap,
+                          int32_t* d_len, uint32_t* d_scratch, int64_t scratch_words, orx_stream_t stream);
 
 /* ---- Per-arc accountant on the CSR arc order (EdgeAccountant, UP:1039-1422).
  * Device arrays of E entries indexed by CSR arc id.  The arithmetic is the

@@ -309,10 +309,12 @@ def sssp_box(spec, wcost: np.ndarray, box, src: int, dst: int = -1, full: bool =
 
 
 def sssp_csr(N, indptr, indices, costs, in_rank, in_indptr, in_src, roi_nodes=None, g2r=None, pen=None,
-             allowed=None, seeds=(), seed_cost=None, targets=(), tg_cost=None, seed_pen=True):
-    """Generic CSR query: returns (rc, best_target, dist[R], parent_roi[R]).
-    seed_pen: seeds pay their own node penalty (multi-seed forms); the
-    single-source find_path_roi form passes False."""
+             allowed=None, seeds=(), seed_cost=None, targets=(), tg_cost=None, seed_pen=True, counts=False):
+    """Generic CSR query: returns (rc, best_target, dist[R], parent_roi[R]),
+    and with ``counts`` also (settled, relaxed): the nodes the heap search
+    settled and the arcs it relaxed.  seed_pen: seeds pay their own node
+    penalty (multi-seed forms); the single-source find_path_roi form passes
+    False."""
     lib = load()
     a = lambda x, t: None if x is None else np.ascontiguousarray(x, t)  # noqa: E731
     indptr, indices = a(indptr, np.int32), a(indices, np.int32)
@@ -334,6 +336,8 @@ def sssp_csr(N, indptr, indices, costs, in_rank, in_indptr, in_src, roi_nodes=No
                                  _p(best), _p(stats))
     if rc < 0:
         raise RuntimeError(f"oracle sssp_csr failed: {rc}")
+    if counts:
+        return rc, int(best[0]), dist, parent, (int(stats[0]), int(stats[1]))
     return rc, int(best[0]), dist, parent
 
 

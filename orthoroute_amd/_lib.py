@@ -130,6 +130,8 @@ SIGNATURES = [
     ("orx_sssp_csr", C.c_int, [C.POINTER(OrxCsr), P, C.c_int64, P, P, P, P, C.c_int32, P, P, C.c_int32, P, P,
                                P, P, C.c_int64, P, C.POINTER(OrxRouteCfg), P]),
     ("orx_backtrace_csr", C.c_int, [C.POINTER(OrxCsr), P, P, P, P, C.c_int32, P, P]),
+    ("orx_backtrace_csr_roi", C.c_int, [C.POINTER(OrxCsr), P, C.c_int64, P, P, P, P, C.c_int32, P, P, C.c_int64,
+                                        P]),
     ("orx_acct_apply", C.c_int, [C.POINTER(OrxAcct), P, C.c_int64, C.c_int32, C.c_int32, P, P]),
     ("orx_acct_overuse", C.c_int, [C.POINTER(OrxAcct), P, P, P, P]),
     ("orx_acct_sum", C.c_int, [C.POINTER(OrxAcct), C.c_int32, P, P]),

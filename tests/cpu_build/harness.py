@@ -212,3 +212,41 @@ def roi_bfs_host_threads(indptr, indices, N: int, src: int, dst: int, workdir: P
     out = np.fromfile(d / "out.out", np.int64)
     return (np.fromfile(d / "roi.out", np.int32, int(out[0])), np.fromfile(d / "g2r.out", np.int32, N), out,
             np.fromfile(d / "clean.out", np.int64))
+
+
+CSR_DRIVER = HERE / "_out" / "csr_driver"
+
+
+def csr_search_host_threads(N, indptr, indices, in_rank, in_indptr, in_src, costs, seeds, targets, workdir: Path,
+                            seed_cost=None, tg_cost=None, roi_nodes=None, g2r=None, pen=None, grid: bool = True,
+                            walk: bool = True, seed_pen: bool = True, delta: float = 2.0, block: int = 256,
+                            timeout: float = 600.0):
+    """One orx_sssp_csr query and its parent walk through the host-thread
+    build: (keys[R] u64, best, path list)."""
+    d = Path(workdir)
+    d.mkdir(parents=True, exist_ok=True)
+    R = N if roi_nodes is None else len(roi_nodes)
+    meta = {"N": N, "E": len(indices), "R": R, "n_seeds": len(seeds), "n_tg": len(targets),
+            "flags": (8 if grid else 0) | (4 if seed_pen else 0), "block": block, "delta": delta,
+            "roi": int(roi_nodes is not None), "pen": int(pen is not None), "seed_cost": int(seed_cost is not None),
+            "tg_cost": int(tg_cost is not None), "walk": int(walk)}
+    with open(d / "meta.txt", "w") as f:
+        for k, v in meta.items():
+            f.write(f"{k} {float(v)!r}\n")
+    arrays = {"indptr": (indptr, np.int32), "indices": (indices, np.int32), "in_rank": (in_rank, np.int32),
+              "in_indptr": (in_indptr, np.int32), "in_src": (in_src, np.int32), "costs": (costs, np.float32),
+              "seeds": (seeds, np.int32), "targets": (targets, np.int32), "seed_cost": (seed_cost, np.float32),
+              "tg_cost": (tg_cost, np.float32), "roi_nodes": (roi_nodes, np.int32), "g2r": (g2r, np.int32),
+              "pen": (pen, np.float32)}
+    for k, (a, t) in arrays.items():
+        if a is not None:
+            np.ascontiguousarray(a, t).tofile(d / k)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
+    res = subprocess.run([str(CSR_DRIVER), str(d)], capture_output=True, text=True, timeout=timeout, env=env)
+    if res.returncode != 0:
+        raise RuntimeError(f"csr_driver failed ({res.returncode}):\n{res.stderr[-6000:]}")
+    keys = np.fromfile(d / "keys.out", np.uint64, R)
+    best = int(np.fromfile(d / "best.out", np.int32, 1)[0])
+    n = int(np.fromfile(d / "len.out", np.int32, 1)[0])
+    path = np.fromfile(d / "path.out", np.int32, R)[:max(n, 0)].tolist() if n > 1 else []
+    return keys, best, n, path

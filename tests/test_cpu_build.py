@@ -406,3 +406,78 @@ def test_broken_plan_is_refused(driver, tmp_path, mode):
     k = int(np.flatnonzero(st == -1)[0])
     assert c.path_len[k] == 0
     assert d["failed"] >= 1
+
+
+def _lattice_csr(X, Y, L, seed):
+    from orthoroute_amd.lattice import LatticeSpec, in_rank_and_transpose
+    spec = LatticeSpec(X, Y, L)
+    ip, ix = spec.to_csr()[:2]
+    rng = np.random.default_rng(seed)
+    costs = rng.uniform(1.0, 4.0, len(ix)).astype(np.float32)
+    costs[rng.random(len(ix)) < 0.05] *= 10
+    rank, in_ip, in_src = in_rank_and_transpose(spec.N, ip, ix)
+    return spec, np.asarray(ip), np.asarray(ix), costs, rank, in_ip, in_src
+
+
+@pytest.mark.parametrize("conc,roi", [(1, False), (4, True)])
+def test_csr_whole_chip_search_on_host_threads(driver, tmp_path, monkeypatch, conc, roi):
+    """The whole-chip form of the CSR seam search (ORX_F_GRID: grid-wide
+    near-far steps as a device state machine) and the parallel parent walk,
+    on a lattice CSR with random costs: multi-seed / multi-target with seed
+    and target costs, node penalties, whole graph and an ROI map.  Keys'
+    distance bits, the chosen target and the path equal the oracle's."""
+    from oracle.oracle import csr_path, sssp_csr
+    from tests.cpu_build.harness import csr_search_host_threads
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", str(conc))
+    spec, ip, ix, costs, rank, in_ip, in_src = _lattice_csr(16, 14, 3, 3)
+    rng = np.random.default_rng(7)
+    seeds = rng.choice(spec.N, 5, replace=False).astype(np.int32)
+    targets = rng.choice(spec.N, 7, replace=False).astype(np.int32)
+    sc = rng.uniform(0, 3, len(seeds)).astype(np.float32)
+    tc = rng.uniform(0, 3, len(targets)).astype(np.float32)
+    roi_nodes = g2r = None
+    R = spec.N
+    if roi:
+        keep = np.sort(np.unique(np.concatenate([rng.choice(spec.N, spec.N * 3 // 4, replace=False), seeds,
+                                                 targets]))).astype(np.int32)
+        roi_nodes = keep
+        g2r = np.full(spec.N, -1, np.int32)
+        g2r[keep] = np.arange(len(keep), dtype=np.int32)
+        R = len(keep)
+    pen = rng.uniform(0, 0.5, R).astype(np.float32)
+    keys, best, n, path = csr_search_host_threads(spec.N, ip, ix, rank, in_ip, in_src, costs, seeds, targets,
+                                                  tmp_path, seed_cost=sc, tg_cost=tc, roi_nodes=roi_nodes, g2r=g2r,
+                                                  pen=pen, grid=True, walk=True)
+    rc, obest, od, op = sssp_csr(spec.N, ip, ix, costs, rank, in_ip, in_src, roi_nodes=roi_nodes, g2r=g2r, pen=pen,
+                                 seeds=seeds, seed_cost=sc, targets=targets, tg_cost=tc)
+    assert best == obest
+    opath = csr_path(obest, op, roi_nodes, g2r)
+    assert path == opath
+    # every key the oracle settled below the chosen target's total is final
+    dist = (keys >> np.uint64(32)).astype(np.uint32).view(np.float32)
+    fin = np.isfinite(od) & (od < od[(g2r[obest] if roi else obest)])
+    assert np.array_equal(dist[fin].view(np.uint32), od[fin].view(np.uint32))
+
+
+def test_csr_whole_chip_full_search_and_one_lane_walk_agree(driver, tmp_path, monkeypatch):
+    """A single-source whole-graph search to completion (no targets) on the
+    whole-chip form: every node's distance bits equal the oracle's; the
+    parallel walk and the one-lane walk give the same path to a far node."""
+    from oracle.oracle import csr_path, sssp_csr
+    from tests.cpu_build.harness import csr_search_host_threads
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "3")
+    spec, ip, ix, costs, rank, in_ip, in_src = _lattice_csr(24, 18, 3, 5)
+    src = np.array([spec.node(3, 4, 0)], np.int32)
+    far = np.array([spec.node(22, 16, 2)], np.int32)
+    k_grid, b1, n1, p1 = csr_search_host_threads(spec.N, ip, ix, rank, in_ip, in_src, costs, src, far,
+                                                 tmp_path / "a", grid=True, walk=True, seed_pen=False)
+    k_one, b2, n2, p2 = csr_search_host_threads(spec.N, ip, ix, rank, in_ip, in_src, costs, src, far,
+                                                tmp_path / "b", grid=False, walk=False, seed_pen=False)
+    assert b1 == b2 == int(far[0]) and p1 == p2 and n1 == n2 == len(p1) > 33
+    rc, ob, od, op = sssp_csr(spec.N, ip, ix, costs, rank, in_ip, in_src, seeds=src, targets=far, seed_pen=False)
+    assert p1 == csr_path(ob, op)
+    k_all, _, _, _ = csr_search_host_threads(spec.N, ip, ix, rank, in_ip, in_src, costs, src, np.zeros(0, np.int32),
+                                             tmp_path / "c", grid=True, walk=False, seed_pen=False)
+    rc, _, od_all, _ = sssp_csr(spec.N, ip, ix, costs, rank, in_ip, in_src, seeds=src, targets=(), seed_pen=False)
+    dist = (k_all >> np.uint64(32)).astype(np.uint32).view(np.float32)
+    assert np.array_equal(dist.view(np.uint32), od_all.view(np.uint32))
