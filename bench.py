@@ -360,7 +360,7 @@ def serving_bench(n_jobs: int, rate: float, seed: int, max_iterations: int, tune
     gen_s = time.perf_counter() - t0
     cfg = RouterConfig(iterations=max_iterations, stop_at_zero_overuse=True, delta=tuned["delta"],
                        block=tuned["block"], lds_nodes=tuned["lds_nodes"])
-    srv = RoutingServer(cfg, max_slots=8)
+    srv = RoutingServer(cfg, max_slots=8, progress=_log)
     summ = srv.run(jobs, time_limit=time_limit)
     summ.update({
         "arrival_rate_jobs_per_s": rate, "seed": seed, "max_iterations": max_iterations,
@@ -549,6 +549,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-iterations", type=int, default=50)
     ap.add_argument("--serve-seed", type=int, default=4)
+    ap.add_argument("--serve-time-limit", type=float, default=300.0)
     ap.add_argument("--no-roi", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--converge-iterations", type=int, default=30)
@@ -698,7 +699,7 @@ def main() -> int:
     if not args.no_serve:
         _log(f"serving: {args.serve_jobs} jobs at {args.serve_rate} jobs/s")
         serving = serving_bench(args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_iterations, tuned,
-                                time_limit=300.0)
+                                time_limit=args.serve_time_limit)
         torch.cuda.empty_cache()
 
     roi = None

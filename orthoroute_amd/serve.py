@@ -101,7 +101,8 @@ class RoutingServer:
 
     def __init__(self, cfg: RouterConfig | None = None, hbm_budget: int | None = None, max_slots: int = 8,
                  device="cuda", runner: Callable | None = None, streams: list | None = None,
-                 clock: Callable[[], float] = time.perf_counter, idle_sleep: float = 2e-4):
+                 clock: Callable[[], float] = time.perf_counter, idle_sleep: float = 2e-4,
+                 progress: Callable[[str], None] | None = None):
         self.cfg = cfg or RouterConfig(iterations=50, stop_at_zero_overuse=True)
         self.runner = runner or _device_runner
         if streams is None:
@@ -122,6 +123,7 @@ class RoutingServer:
         self.peak_active = 0
         self.clock = clock
         self.idle_sleep = idle_sleep
+        self.progress = progress      # called with a line per retired job
         self._t0 = None
 
     # -- bookkeeping ---------------------------------------------------------
@@ -169,6 +171,10 @@ class RoutingServer:
         self.in_use -= job.footprint
         self.free_slots.append(slot)
         self.finished.append(job)
+        if self.progress is not None:
+            self.progress(f"job {job.job_id} {job.netlist.name} done at {job.finished_at:.1f} s: "
+                          f"latency {job.latency:.1f} s, {job.iterations} iterations, "
+                          f"{len(self.finished)} finished, {len(self.active)} resident, {len(self.pending)} waiting")
 
     def _poll(self) -> bool:
         progressed = False
