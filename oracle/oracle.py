@@ -17,13 +17,17 @@ Parity status of each oracle function (how it is pinned, see tests/):
   accountant (commit / clear / overuse / history / cost) -- pinned:
       tests/unit/test_edge_accountant.py, tests/test_via_accounting.py,
       tests/test_lattice.py:255-276 vectors, restated;
-  net order (_order_nets_by_difficulty) -- host code shared with the product
-      (orthoroute_amd.waves), pinned by construction tests.
+  iteration host logic (difficulty order, present-factor schedule, A* bounds,
+      wave colouring) -- restated below, not imported from the product;
+      pinned by hand-worked orders (the congestion term included), exact
+      fp32 schedules, A*-off runs giving the same state, and the product's
+      own functions checked against these (tests/test_oracle_order.py).
 """
 from __future__ import annotations
 
 import ctypes as C
 import heapq
+import random
 import subprocess
 from pathlib import Path
 
@@ -123,12 +127,114 @@ def params_from(cfg, pres: float) -> OracleParams:
                         cfg.hist_cap_mult, cfg.node_hist_gain)
 
 
+# -- host logic of one iteration, restated from the reference ----------------
+
+def lattice_distance(X: int, Y: int, src, dst) -> np.ndarray:
+    """Manhattan distance in lattice coordinates between each net's source and
+    sink (unified_pathfinder.py:7851-7854; node = (z*Y + y)*X + x)."""
+    out = np.zeros(len(src), np.int64)
+    for k, (s, t) in enumerate(zip(src, dst)):
+        sz, r = divmod(int(s), X * Y)
+        sy, sx = divmod(r, X)
+        tz, r = divmod(int(t), X * Y)
+        ty, tx = divmod(r, X)
+        out[k] = abs(tx - sx) + abs(ty - sy) + abs(tz - sz)
+    return out
+
+
+def difficulty_order(iteration: int, distance, over_sum, hops) -> np.ndarray:
+    """Hardest-first routing order of iteration ``iteration`` (1-based),
+    _order_nets_by_difficulty (unified_pathfinder.py:7840-7900) without barrel
+    roles: difficulty = distance x (pin degree 2 + 1) x (congestion + 1), where
+    congestion is the mean overuse max(0, present - cap) over the arcs of the
+    net's previous path (``over_sum`` / ``hops``; 0 with no path); sorted on
+    (difficulty, net id) descending, then rotated left by
+    random.Random(42 + iteration).randint(0, min(5, n // 10))."""
+    scores = []
+    for k in range(len(distance)):
+        congestion = float(over_sum[k]) / max(1, int(hops[k]))
+        scores.append((int(distance[k]) * (2 + 1) * (congestion + 1), k))
+    scores.sort(reverse=True)
+    ordered = [k for _, k in scores]
+    rotation = random.Random(42 + int(iteration)).randint(0, min(5, len(scores) // 10))
+    if rotation > 0:
+        ordered = ordered[rotation:] + ordered[:rotation]
+    return np.asarray(ordered, np.int32)
+
+
+def present_factor_max(L: int, configured: float = 64.0) -> float:
+    """resolve_pres_fac_max (unified_pathfinder.py:801-814) over the L - 2
+    signal layers: the configured ceiling, at least 32 / 64 / 128 for up to
+    12 / 20 / more signal layers."""
+    signal = L - 2
+    floor = 32.0 if signal <= 12 else 64.0 if signal <= 20 else 128.0
+    return max(float(configured), floor)
+
+
+def present_factors(iterations: int, init: float, mult: float, pmax: float) -> np.ndarray:
+    """fp32 present factor of each iteration: init, then min(p x mult, pmax)
+    (unified_pathfinder.py:7379-7383 at work scale 1)."""
+    out = []
+    p = np.float32(init)
+    for _ in range(iterations):
+        out.append(p)
+        p = min(np.float32(p * np.float32(mult)), np.float32(pmax))
+    return np.asarray(out, np.float32)
+
+
+def astar_lower_bounds(base: np.ndarray, base_w: float, margin_bits: int = 8) -> tuple[float, float, float]:
+    """(cp, cv, gmax) of the sink-directed search: the cheapest finite planar
+    (slots 1, 2) and via (slots 0, 3) entering weight fl(b x base_w), shrunk
+    by 2^-margin_bits and rounded down in fp32, and the distance below which
+    that margin covers half an fp32 ulp, 2^(24 + floor(log2 margin)).  Paths
+    do not depend on them (an A*-off run gives the same state); the search
+    counts do."""
+    b = np.asarray(base, np.float32).reshape(-1, 4)
+    bw = np.float32(base_w)
+    mins = []
+    for cols in ((1, 2), (0, 3)):
+        v = b[:, cols].ravel()
+        v = v[np.isfinite(v)]
+        mins.append(float(np.float32(v.min()) * bw) if v.size else None)
+    if mins[0] is None:
+        return (0.0, 0.0, 0.0)
+    if mins[1] is None:
+        mins[1] = mins[0]
+    if min(mins) <= 0:
+        return (0.0, 0.0, 0.0)
+    shrunk = []
+    for m in mins:
+        f = np.float32(m * (1.0 - 2.0 ** -margin_bits))
+        while float(f) > m * (1.0 - 2.0 ** -margin_bits):
+            f = np.nextafter(f, np.float32(0))
+        shrunk.append(f)
+    margin = min(m - float(f) for m, f in zip(mins, shrunk))
+    gmax = 2.0 ** min(24 + int(np.floor(np.log2(margin))), 120)
+    return (float(shrunk[0]), float(shrunk[1]), float(gmax))
+
+
+def disjoint_waves(box: np.ndarray, order: np.ndarray) -> list[np.ndarray]:
+    """Nets of ``order`` in waves of pairwise disjoint boxes: each net goes one
+    wave after the latest earlier net whose box overlaps its own, so nets
+    that share cells keep their order and a wave's nets can route in any
+    order.  Inside a wave nets keep the routing order."""
+    b = np.asarray(box, np.int64).reshape(-1, 4)
+    order = np.asarray(order, np.int64)
+    wave = np.full(len(order), -1, np.int64)
+    x0, y0 = b[order, 0], b[order, 1]
+    x1, y1 = x0 + b[order, 2], y0 + b[order, 3]
+    for i in range(len(order)):
+        hit = (x0[:i] < x1[i]) & (x0[i] < x1[:i]) & (y0[:i] < y1[i]) & (y0[i] < y1[:i])
+        wave[i] = int(wave[:i][hit].max()) + 1 if hit.any() else 0
+    n_waves = int(wave.max()) + 1 if len(order) else 0
+    return [order[wave == w].astype(np.int32) for w in range(n_waves)]
+
+
 class OraclePathFinder:
-    """Sequential CPU PathFinder over box ROIs with the same host order logic
-    as the device driver (orthoroute_amd.router.DevicePathFinder)."""
+    """Sequential CPU PathFinder over box ROIs.  Its host logic (order,
+    present factors, A* bounds) is the restatement above, not the product's."""
 
     def __init__(self, netlist, cfg, base: np.ndarray | None = None):
-        from orthoroute_amd.waves import net_distance
         self.lib = load()
         self.netlist = netlist
         self.cfg = cfg
@@ -152,9 +258,13 @@ class OraclePathFinder:
         self.box = np.ascontiguousarray(netlist.box.reshape(-1), np.int32)
         self.src = netlist.src.astype(np.int32)
         self.dst = netlist.dst.astype(np.int32)
-        self.distance = net_distance(spec, netlist.src, netlist.dst)
-        self.schedule = cfg.schedule(spec.L)
-        self.astar = cfg.astar_bounds(self.base)
+        self.distance = lattice_distance(spec.X, spec.Y, netlist.src, netlist.dst)
+        pmax = cfg.pres_max if cfg.pres_max is not None else present_factor_max(spec.L)
+        self.schedule = present_factors(cfg.iterations, cfg.pres_init, cfg.pres_mult, pmax)
+        weights = (cfg.base_w, cfg.hist_w, cfg.node_pres_w, cfg.node_hist_w, cfg.hist_gain, cfg.node_hist_gain,
+                   cfg.hist_decay)
+        astar_ok = cfg.astar and min(weights) >= 0 and cfg.pres_init >= 0 and cfg.pres_mult >= 0
+        self.astar = astar_lower_bounds(self.base, cfg.base_w) if astar_ok else (0.0, 0.0, 0.0)
         self.iteration = 0
         self.history = []
         self.terms, self._terms_keep = terms_struct(getattr(netlist, "terms", None))
@@ -169,13 +279,12 @@ class OraclePathFinder:
         return over, hops
 
     def order_for(self, iteration: int) -> np.ndarray:
-        from orthoroute_amd.waves import net_order
         if iteration > 1:
             over, hops = self.net_overuse()
         else:
             over = np.zeros(self.netlist.n, np.int64)
             hops = np.zeros(self.netlist.n, np.int32)
-        order = net_order(iteration, self.distance, over, hops)
+        order = difficulty_order(iteration, self.distance, over, hops)
         h = self.history
         stalled = len(h) >= 2 and (h[-1]["edge_over"] + h[-1]["node_over"] >= h[-2]["edge_over"] + h[-2]["node_over"])
         if getattr(self.cfg, "hotset", False) and iteration > 1 and not stalled:
@@ -220,11 +329,12 @@ class OraclePathFinder:
         """The same iteration as run_iteration, with the nets of each
         disjoint-ROI wave routed on `threads` host threads (0 = all); the
         result equals the sequential order's."""
-        from orthoroute_amd.waves import color_waves, group_waves
         it = self.iteration + 1
         order = self.order_for(it)
         s = self.spec
-        waves = group_waves(color_waves(self.netlist.box, order, s.X, s.Y), order, self.netlist.roi_nodes())
+        roi = self.netlist.roi_nodes()
+        # largest ROI first inside a wave: the host threads start the long searches first
+        waves = [w[np.argsort(-roi[w], kind="stable")] for w in disjoint_waves(self.netlist.box, order)]
         sched = np.ascontiguousarray(np.concatenate(waves) if waves else np.zeros(0, np.int32), np.int32)
         wave_ptr = np.zeros(len(waves) + 1, np.int32)
         np.cumsum([len(w) for w in waves], out=wave_ptr[1:])
