@@ -774,15 +774,18 @@ __global__ __launch_bounds__(BLOCK) void reprice_paths_kernel(Lat g, Params P, c
 // Per-net overuse over the forward arcs of its committed path: the
 // numerator of _order_nets_by_difficulty's congestion term
 // (unified_pathfinder.py:7858-7863; both arcs of a hop hold equal occupancy).
+// One wavefront per net, its lanes striding over the path's hops, so a long
+// path costs len / 64 dependent loads instead of len.
 __global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, int32_t edge_cap, int32_t node_cap,
                                                           int64_t* out_over, int64_t* out_node_over,
                                                           int32_t* out_hops) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nets.n) return;
+    const int32_t k = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
+    if (k >= nets.n) return;   // uniform per wavefront
     const int32_t len = nets.path_len[k];
     const int32_t* path = nets.path_arena + nets.path_off[k];
     int64_t over = 0, node_over = 0;
-    for (int32_t i = 0; i < len; ++i) {
+    for (int32_t i = lane; i < len; i += 64) {
         const int64_t u = path[i];
         if (out_node_over) {
             const int32_t o = g.node_use[u] - node_cap;
@@ -794,9 +797,15 @@ __global__ __launch_bounds__(256) void net_overuse_kernel(Lat g, NetsDev nets, i
             over += o > 0 ? o : 0;
         }
     }
-    out_over[k] = over;
-    if (out_node_over) out_node_over[k] = node_over;
-    out_hops[k] = len > 0 ? len - 1 : 0;
+    for (int m = 32; m >= 1; m >>= 1) {
+        over += __shfl_xor(over, m);
+        node_over += __shfl_xor(node_over, m);
+    }
+    if (lane == 0) {
+        out_over[k] = over;
+        if (out_node_over) out_node_over[k] = node_over;
+        out_hops[k] = len > 0 ? len - 1 : 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1494,7 +1503,7 @@ int orx_net_overuse(const orx_lattice* lat, const orx_nets* nets, int32_t edge_c
     if (!nets || !d_over || !d_hops) return orx_host::fail(ORX_E_ARG, "NULL argument");
     if (nets->n <= 0) return ORX_OK;
     const NetsDev nd = make_nets(nets);
-    hipLaunchKernelGGL(net_overuse_kernel, dim3((nets->n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(net_overuse_kernel, dim3((nets->n + 3) / 4), dim3(256), 0, (hipStream_t)stream,
                        make_lat(lat), nd, edge_cap, node_cap, d_over, d_node_over, d_hops);
     ORX_HIP_CHECK(hipGetLastError());
     return ORX_OK;

@@ -277,88 +277,62 @@ __device__ void box_stats(const Lat& g, const Box& B, const Heur& H, const uint6
 }
 
 
-// Octahedron of radius 3 around a node: the 63 offsets (dx, dy, dz) with
-// |dx| + |dy| + |dz| <= 3 (every node three parent moves can reach), and
-// the lane of each offset, indexed (dx+3)*49 + (dy+3)*7 + (dz+3) (-1 = none).
-__device__ __constant__ int8_t kStencilDx[63] = {
-    0, 0, -1, 0, 1, 0, 0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0, 0, -1,
-    0, 1, -2, -1, 0, 1, 2, -3, -2, -1, 0, 1, 2, 3, -2, -1, 0, 1, 2, -1, 0,
-    1, 0, 0, -1, 0, 1, -2, -1, 0, 1, 2, -1, 0, 1, 0, 0, -1, 0, 1, 0, 0,
-};
-__device__ __constant__ int8_t kStencilDy[63] = {
-    0, -1, 0, 0, 0, 1, -2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2, -3, -2,
-    -2, -2, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2,
-    2, 3, -2, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 2, -1, 0, 0, 0, 1, 0,
-};
-__device__ __constant__ int8_t kStencilDz[63] = {
-    -3, -2, -2, -2, -2, -2, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 0,
-    0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
-    0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 3,
-};
-__device__ __constant__ int8_t kStencilLane[343] = {
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 28, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 23, -1, -1, -1, -1, -1, 10, 29, 48, -1, -1, -1, -1, -1, 35, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 20, -1, -1, -1, -1, -1, 7, 24, 45, -1, -1, -1, 2, 11, 30, 49, 58, -1, -1, -1, 15, 36, 53, -1, -1, -1, -1, -1, 40, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
-    -1, -1, -1, 19, -1, -1, -1, -1, -1, 6, 21, 44, -1, -1, -1, 1, 8, 25, 46, 57, -1, 0, 3, 12, 31, 50, 59, 62, -1, 5, 16, 37, 54, 61, -1, -1, -1, 18, 41, 56, -1, -1, -1, -1, -1, 43, -1, -1, -1,
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 22, -1, -1, -1, -1, -1, 9, 26, 47, -1, -1, -1, 4, 13, 32, 51, 60, -1, -1, -1, 17, 38, 55, -1, -1, -1, -1, -1, 42, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 27, -1, -1, -1, -1, -1, 14, 33, 52, -1, -1, -1, -1, -1, 39, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 34, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
-};
-
-// Parent walk by one wavefront.  The lanes load the keys of the 63 nodes
-// within three moves of the current node at once; then every lane follows
-// up to four parent links through shuffles (the nodes it looks up are at
-// most three moves away; the same walk on every lane, so control flow stays
-// uniform).  One dependent memory round trip per four hops instead of one
-// per hop.  Lane 0 writes the path, root first.  The walk must end at rsrc,
+// Parent walk by one wavefront, a straight run per memory round trip.  A
+// node's parent lies one move away in the direction its in-slot names, and a
+// shortest path on the layered lattice is a few long straight runs (a turn
+// costs a via).  So from the current node the 64 lanes load the keys of the
+// next 64 nodes in its parent direction at once; the run continues through
+// every node whose in-slot is the same (same slot, same direction: planar
+// moves keep their layer), and the first node whose slot differs (a turn or
+// the root) ends the trip and starts the next one.  Lane j writes the path
+// entry of the j-th node of the run, root first.  The walk must end at rsrc,
 // or at any root (a seed) when rsrc is UINT32_MAX.  Returns the path length,
 // 0 when rdst was not reached, or a negative status.
 // Call with all 64 lanes of one wave; every lane returns the same value.
 __device__ int32_t box_backtrace_wave(const Lat& g, const Box& B, const uint64_t* keys, uint32_t rsrc,
                                       uint32_t rdst, int32_t* path, int32_t cap) {
-    const uint32_t lane = lane_id();
+    const int32_t lane = (int32_t)lane_id();
     const uint64_t kd = key_load(keys + rdst);
     if (kd == ORX_KEY_INF) return 0;
     const uint32_t hops = khops(kd);
     if ((int64_t)hops + 1 > cap) return ORX_E_OVERFLOW;
-    const int32_t sdx = lane < 63 ? kStencilDx[lane] : 0;
-    const int32_t sdy = lane < 63 ? kStencilDy[lane] : 0;
-    const int32_t sdz = lane < 63 ? kStencilDz[lane] : 0;
     uint32_t ux, uy, uz;
     roi_decode(B, rdst, ux, uy, uz);
     int32_t cx = (int32_t)ux, cy = (int32_t)uy, cz = (int32_t)uz;
-    int32_t i = (int32_t)hops;
+    int32_t i = (int32_t)hops;          // path index of the current node
+    uint32_t sl = kslot(kd);            // its in-slot
+    if (lane == 0) path[i] = (int32_t)roi_global(g, B, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
     for (;;) {
-        const int32_t nx = cx + sdx, ny = cy + sdy, nz = cz + sdz;
-        uint64_t mine = ORX_KEY_INF;
-        if (lane < 63 && nx >= 0 && nx < B.w && ny >= 0 && ny < B.h && nz >= 0 && nz < B.L)
-            mine = key_load(keys + ((uint32_t)nz * (uint32_t)B.h + (uint32_t)ny) * (uint32_t)B.w + (uint32_t)nx);
-        int32_t ox = 0, oy = 0, oz = 0;
-        for (int step = 0; step < 4; ++step) {
-            const int li = kStencilLane[(ox + 3) * 49 + (oy + 3) * 7 + (oz + 3)];
-            const uint64_t k = __shfl(mine, li);
-            const int32_t vx = cx + ox, vy = cy + oy, vz = cz + oz;
-            const uint32_t v = ((uint32_t)vz * (uint32_t)B.h + (uint32_t)vy) * (uint32_t)B.w + (uint32_t)vx;
-            if (lane == 0) path[i] = (int32_t)roi_global(g, B, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
-            const uint32_t sl = kslot(k);
-            if (i == 0) return (sl == ORX_ROOT_SLOT && (rsrc == UINT32_MAX || v == rsrc)) ? (int32_t)hops + 1 : ORX_E_CYCLE;
-            if (sl > 3) return ORX_E_CYCLE;
-            int32_t mx = 0, my = 0, mz = 0;
-            if (sl == 0) {
-                mz = -1;
-            } else if (sl == 3) {
-                mz = 1;
-            } else {
-                const int32_t d = sl == 1 ? -1 : 1;
-                if ((g.hmask >> vz) & 1u) mx = d;
-                else my = d;
-            }
-            const int32_t px = vx + mx, py = vy + my, pz = vz + mz;
-            if (px < 0 || px >= B.w || py < 0 || py >= B.h || pz < 0 || pz >= B.L) return ORX_E_CYCLE;
-            ox += mx; oy += my; oz += mz;
-            --i;
+        if (i == 0) {
+            const uint32_t v = ((uint32_t)cz * (uint32_t)B.h + (uint32_t)cy) * (uint32_t)B.w + (uint32_t)cx;
+            return (sl == ORX_ROOT_SLOT && (rsrc == UINT32_MAX || v == rsrc)) ? (int32_t)hops + 1 : ORX_E_CYCLE;
         }
-        cx += ox; cy += oy; cz += oz;
+        if (sl > 3) return ORX_E_CYCLE;
+        int32_t mx = 0, my = 0, mz = 0;
+        if (sl == 0) {
+            mz = -1;
+        } else if (sl == 3) {
+            mz = 1;
+        } else {
+            const int32_t d = sl == 1 ? -1 : 1;
+            if ((g.hmask >> cz) & 1u) mx = d;
+            else my = d;
+        }
+        // lane j: the node j + 1 moves ahead
+        const int32_t nx = cx + (lane + 1) * mx, ny = cy + (lane + 1) * my, nz = cz + (lane + 1) * mz;
+        const bool inside = nx >= 0 && nx < B.w && ny >= 0 && ny < B.h && nz >= 0 && nz < B.L;
+        uint64_t k = ORX_KEY_INF;
+        if (inside) k = key_load(keys + ((uint32_t)nz * (uint32_t)B.h + (uint32_t)ny) * (uint32_t)B.w + (uint32_t)nx);
+        // the run ends at the first lane that is outside the box or turns
+        const uint64_t brk = __ballot(!inside || kslot(k) != sl);
+        const int32_t b = brk ? (int32_t)__ffsll((long long)brk) - 1 : 64;
+        if (b < 64 && !((__ballot(inside) >> b) & 1ull)) return ORX_E_CYCLE;   // a parent outside the box
+        const int32_t take = b < 64 ? b + 1 : 64;   // nodes of the run, the turning node included
+        if (take > i) return ORX_E_CYCLE;          // longer than the target's hop count
+        if (lane < take) path[i - 1 - lane] = (int32_t)roi_global(g, B, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz);
+        sl = kslot(__shfl(k, take - 1));
+        cx += take * mx; cy += take * my; cz += take * mz;
+        i -= take;
     }
 }
 
