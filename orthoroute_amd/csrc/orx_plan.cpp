@@ -80,20 +80,26 @@ extern "C" int64_t orx_plan_iteration(const int32_t* box, const int32_t* order, 
 // Waves only (no dependency lists), for the per-iteration launch plan.
 // wave[k] = 1 + the largest wave of any earlier net (in `order`) whose box
 // overlaps net k's box -- the same waves as orx_plan_iteration -- computed on
-// a two-level grid: 8x8-cell tiles keep the largest wave of boxes covering
-// the whole tile (tfull) and the largest wave anywhere in the tile (tmax),
-// so a box visits its fully covered tiles once each and only the cells of
-// the tiles its border crosses.  A cell's wave is max(cell, tfull of its
-// tile).  Every box's wave exceeds every value inside it, so an update only
-// ever raises values.
+// 8x8-cell tiles.  Each tile keeps the wave of the last box covering all of
+// it (tfull), the largest wave anywhere in it (tmax), and the boxes that
+// covered part of it since as up to K stamps (a rectangle and its wave);
+// when the stamps are full they are written into the tile's cells.  A box
+// visits each tile it touches once to read (skipped when tmax is no larger
+// than what it has found) and once to write (one stamp, or tfull).  The
+// wave of a cell is the max of tfull, the stamps covering it and the cell.
+// Every box's wave exceeds every value inside it, so values only rise.
 extern "C" int64_t orx_plan_waves(const int32_t* box, int32_t n_nets, const int32_t* order, int32_t n, int32_t X,
                                   int32_t Y, const int64_t* roi_nodes, int32_t* wave, int32_t* sched) {
     if (n < 0 || n > n_nets || X <= 0 || Y <= 0) return -1;
     for (int32_t i = 0; i < n; ++i)
         if (order[i] < 0 || order[i] >= n_nets) return -3;
-    constexpr int32_t T = 8;
+    constexpr int32_t T = 8, K = 8;   // tile side in cells, stamps per tile
+    struct Stamp { uint8_t x0, x1, y0, y1; int32_t w; };
     const int32_t TX = (X + T - 1) / T, TY = (Y + T - 1) / T;
-    std::vector<int32_t> cell((size_t)X * Y, -1), tfull((size_t)TX * TY, -1), tmax((size_t)TX * TY, -1);
+    const size_t NT = (size_t)TX * TY;
+    std::vector<int32_t> cell(NT * T * T, -1), tfull(NT, -1), tmax(NT, -1);
+    std::vector<Stamp> stamps(NT * K);
+    std::vector<uint8_t> ns(NT, 0);
     int32_t nw = 0;
     for (int32_t i = 0; i < n; ++i) {
         const int32_t k = order[i];
@@ -103,43 +109,50 @@ extern "C" int64_t orx_plan_waves(const int32_t* box, int32_t n_nets, const int3
         const int32_t tx0 = x0 / T, tx1 = (x1 - 1) / T, ty0 = y0 / T, ty1 = (y1 - 1) / T;
         int32_t m = -1;
         for (int32_t ty = ty0; ty <= ty1; ++ty) {
-            const int32_t cy0 = std::max(y0, ty * T), cy1 = std::min(y1, ty * T + T);
-            const bool full_y = cy0 == ty * T && cy1 == std::min(Y, ty * T + T) && cy1 - cy0 == std::min(T, Y - ty * T);
+            const int32_t cy0 = std::max(y0, ty * T) - ty * T, cy1 = std::min(y1, ty * T + T) - ty * T;
+            const bool full_y = cy0 == 0 && cy1 == std::min(T, Y - ty * T);
             for (int32_t tx = tx0; tx <= tx1; ++tx) {
                 const size_t t = (size_t)ty * TX + tx;
-                const int32_t cx0 = std::max(x0, tx * T), cx1 = std::min(x1, tx * T + T);
-                const bool full = full_y && cx0 == tx * T && cx1 == std::min(X, tx * T + T);
-                if (full) {
-                    m = std::max(m, tmax[t]);
-                } else if (tmax[t] > m) {
-                    int32_t cm = tfull[t];
-                    for (int32_t y = cy0; y < cy1; ++y) {
-                        const int32_t* row = cell.data() + (size_t)y * X;
-                        for (int32_t x = cx0; x < cx1; ++x) cm = std::max(cm, row[x]);
-                    }
-                    m = std::max(m, cm);
+                if (tmax[t] <= m) continue;
+                const int32_t cx0 = std::max(x0, tx * T) - tx * T, cx1 = std::min(x1, tx * T + T) - tx * T;
+                if (full_y && cx0 == 0 && cx1 == std::min(T, X - tx * T)) {
+                    m = tmax[t];
+                    continue;
                 }
+                int32_t cm = tfull[t];
+                const Stamp* st = stamps.data() + t * K;
+                for (int32_t q = 0; q < ns[t]; ++q)
+                    if (st[q].w > cm && st[q].x0 < cx1 && cx0 < st[q].x1 && st[q].y0 < cy1 && cy0 < st[q].y1) cm = st[q].w;
+                const int32_t* c = cell.data() + t * (T * T);
+                for (int32_t y = cy0; y < cy1; ++y)
+                    for (int32_t x = cx0; x < cx1; ++x) cm = std::max(cm, c[y * T + x]);
+                m = std::max(m, cm);
             }
         }
         const int32_t wk = m + 1;
         wave[k] = wk;
         nw = std::max(nw, wk + 1);
         for (int32_t ty = ty0; ty <= ty1; ++ty) {
-            const int32_t cy0 = std::max(y0, ty * T), cy1 = std::min(y1, ty * T + T);
-            const bool full_y = cy0 == ty * T && cy1 - cy0 == std::min(T, Y - ty * T);
+            const int32_t cy0 = std::max(y0, ty * T) - ty * T, cy1 = std::min(y1, ty * T + T) - ty * T;
+            const bool full_y = cy0 == 0 && cy1 == std::min(T, Y - ty * T);
             for (int32_t tx = tx0; tx <= tx1; ++tx) {
                 const size_t t = (size_t)ty * TX + tx;
-                const int32_t cx0 = std::max(x0, tx * T), cx1 = std::min(x1, tx * T + T);
-                const bool full = full_y && cx0 == tx * T && cx1 - cx0 == std::min(T, X - tx * T);
-                if (full) {
-                    tfull[t] = wk;
-                } else {
-                    for (int32_t y = cy0; y < cy1; ++y) {
-                        int32_t* row = cell.data() + (size_t)y * X;
-                        for (int32_t x = cx0; x < cx1; ++x) row[x] = wk;
-                    }
-                }
+                const int32_t cx0 = std::max(x0, tx * T) - tx * T, cx1 = std::min(x1, tx * T + T) - tx * T;
                 tmax[t] = std::max(tmax[t], wk);
+                if (full_y && cx0 == 0 && cx1 == std::min(T, X - tx * T)) {
+                    tfull[t] = wk;
+                    ns[t] = 0;
+                    continue;
+                }
+                Stamp* st = stamps.data() + t * K;
+                if (ns[t] == K) {
+                    int32_t* c = cell.data() + t * (T * T);
+                    for (int32_t q = 0; q < K; ++q)
+                        for (int32_t y = st[q].y0; y < st[q].y1; ++y)
+                            for (int32_t x = st[q].x0; x < st[q].x1; ++x) c[y * T + x] = std::max(c[y * T + x], st[q].w);
+                    ns[t] = 0;
+                }
+                st[ns[t]++] = Stamp{(uint8_t)cx0, (uint8_t)cx1, (uint8_t)cy0, (uint8_t)cy1, wk};
             }
         }
     }

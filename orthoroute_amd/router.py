@@ -337,6 +337,16 @@ def split_waves(waves: list[np.ndarray], roi_nodes: np.ndarray, cap_nodes: int) 
     """Split every wave whose 64-aligned ROI total exceeds ``cap_nodes`` into
     consecutive parts that fit (nets of one wave are independent, so any
     split gives the same result).  Every ROI must fit on its own."""
+    if not waves:
+        return []
+    sizes = np.fromiter((len(w) for w in waves), np.int64, len(waves))
+    if sizes.min() > 0:
+        # common case, one pass: no wave exceeds the cap
+        r64_all = (roi_nodes[np.concatenate(waves)].astype(np.int64) + 63) // 64 * 64
+        starts = np.zeros(len(waves), np.int64)
+        np.cumsum(sizes[:-1], out=starts[1:])
+        if int(np.add.reduceat(r64_all, starts).max()) <= cap_nodes:
+            return list(waves)
     out = []
     for w in waves:
         r64 = (roi_nodes[w].astype(np.int64) + 63) // 64 * 64

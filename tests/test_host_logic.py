@@ -50,6 +50,22 @@ def test_tiled_planner_equals_cell_planner(seed):
     assert sum(len(w) for w in got) == nl.n and len(got) == int(wave.max()) + 1
 
 
+@pytest.mark.parametrize("seed", [4, 9, 12])
+def test_tiled_planner_on_dense_serving_boards_and_subsets(seed):
+    """Boards whose small boxes cover each tile partly many times (the tiles'
+    stamp lists fill and spill into cells), full and hotset-like orders."""
+    from orthoroute_amd.netgen import serving_job
+    nl = serving_job(np.random.default_rng(seed), 0)
+    R = nl.roi_nodes()
+    rng = np.random.default_rng(seed)
+    for order in (rng.permutation(nl.n), rng.permutation(nl.n)[: nl.n // 3]):
+        order = order.astype(np.int32)
+        want = group_waves(color_waves(nl.box, order, nl.spec.X, nl.spec.Y), order, R)
+        got = plan_waves(nl.box, order, nl.spec.X, nl.spec.Y, R)
+        assert len(got) == len(want)
+        assert all(np.array_equal(a, b) for a, b in zip(got, want))
+
+
 def test_waves_are_disjoint_and_keep_sequential_order():
     nl = uneven(120, 120, 4, 400, seed=11)
     order = np.random.default_rng(3).permutation(nl.n).astype(np.int32)
