@@ -287,6 +287,30 @@ def test_tile_engine_on_host_threads(driver, tmp_path, monkeypatch, X, Y, L, n, 
         assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
 
 
+@pytest.mark.parametrize("phase_cap", ["0", "3"])
+def test_tile_engine_phase_cap_on_host_threads(driver, tmp_path, monkeypatch, phase_cap):
+    """Routing-wave visits with no phase cap and with a cap of three phases
+    (a visit that still changes keys stops and its tile is visited again,
+    rescanned from scratch, in the next step): both equal the oracle's."""
+    from tests.cpu_build.harness import CpuBuildPathFinder
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "4")
+    monkeypatch.setenv("ORX_TILE_PHASE_CAP", phase_cap)
+    nl = uneven(40, 34, 6, 12, seed=6)
+    cfg = RouterConfig(iterations=2)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3, extra_meta={"tile_steps_cap": -1})
+    o = OraclePathFinder(nl, cfg)
+    for _ in range(2):
+        d = c.run_iteration(timeout=600)
+        od = o.run_iteration()
+        for k in ("routed", "failed", "edge_over", "node_over", "settled", "relaxed", "path_hops"):
+            assert d[k] == od[k], k
+        for a, b in zip(c.paths(), o.paths()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(c.path_cost.view(np.uint32), o.path_cost.view(np.uint32))
+        assert np.array_equal(c.occ, o.occ)
+        assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
+
+
 @pytest.mark.parametrize("X,Y,L,n,seed,conc", [
     (40, 34, 6, 16, 4, 1),     # class 6, workgroups one at a time (the first one does every visit)
     (40, 34, 6, 16, 4, 4),     # concurrent workgroups taking tiles from the ring
