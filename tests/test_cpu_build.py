@@ -287,17 +287,18 @@ def test_tile_engine_on_host_threads(driver, tmp_path, monkeypatch, X, Y, L, n, 
         assert np.array_equal(c.hist.view(np.uint32), o.hist.view(np.uint32))
 
 
-@pytest.mark.parametrize("phase_cap", ["0", "3"])
-def test_tile_engine_phase_cap_on_host_threads(driver, tmp_path, monkeypatch, phase_cap):
-    """Routing-wave visits with no phase cap and with a cap of three phases
-    (a visit that still changes keys stops and its tile is visited again,
-    rescanned from scratch, in the next step): both equal the oracle's."""
+def test_tile_engine_phase_cap_on_host_threads(driver, tmp_path, monkeypatch):
+    """Routing-wave visits capped at three phases (a visit that still changes
+    keys stops and its tile is visited again, rescanned from scratch, in the
+    next step) equal the oracle's.  The default cap (four) runs in every
+    other routing test here, the uncapped visit in the whole-box queries."""
     from tests.cpu_build.harness import CpuBuildPathFinder
-    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "4")
-    monkeypatch.setenv("ORX_TILE_PHASE_CAP", phase_cap)
-    nl = uneven(40, 34, 6, 12, seed=6)
+    monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "2")
+    monkeypatch.setenv("ORX_TILE_PHASE_CAP", "3")
+    nl = uneven(30, 26, 6, 12, seed=7)
     cfg = RouterConfig(iterations=2)
-    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3, extra_meta={"tile_steps_cap": -1})
+    # one step launch, then the single-workgroup drain (capped visits go back on its list)
+    c = CpuBuildPathFinder(nl, cfg, tmp_path, mode=3, extra_meta={"tile_steps_cap": 1})
     o = OraclePathFinder(nl, cfg)
     for _ in range(2):
         d = c.run_iteration(timeout=600)
