@@ -67,6 +67,7 @@ class RouterConfig:
     tile_queue: bool = False          # tile engine, queue form: one persistent launch per wave (ORX_F_ASYNC)
     small_wave_nodes: int = 16384     # tile engine: waves whose largest ROI is at most this run one workgroup per net
     tile_groups: int = 0              # tile engine: cap on the workgroups of a launch (0: the whole chip)
+    head_nets: int = 128              # batched: plan and launch the first this many nets of the order before the rest
 
     def schedule(self, L: int) -> np.ndarray:
         pmax = self.pres_max if self.pres_max is not None else resolve_pres_fac_max(L)
@@ -661,30 +662,48 @@ class DevicePathFinder:
         t0 = time.perf_counter()
         if order is None:
             order = self.order_for(it)
+        # Split planning: the head of the order (the hardest nets, whose
+        # waves take the most device time) is planned and launched first, and
+        # the rest is planned while the device routes the head.  The rest's
+        # waves start after the head's, so every net still comes after each
+        # earlier net whose box it overlaps: the same results.
+        rest = None
         if waves is None:
-            waves = self.plan(order, mode)
-        flat_nets, flat_off, wave_ptr, wave_max, wave_ws, need = _flatten_waves(waves, self.nets.roi_nodes)
+            h = int(self.cfg.head_nets)
+            if mode in ("batched", "waves") and not graph and h > 0 and len(order) > 2 * h:
+                waves, rest = self.plan(order[:h], mode), order[h:]
+            else:
+                waves = self.plan(order, mode)
         s = self._s()
         ev = tuple(torch.cuda.Event(enable_timing=True) for _ in range(4))
-        m = len(flat_nets)
-        if m > len(self._plan_nets_h):
-            raise ValueError("a launch plan lists more nets than the job has")
-        self._plan_nets_h.numpy()[:m] = flat_nets
-        self._plan_off_h.numpy()[:m] = flat_off
-        with torch.cuda.stream(stream):
-            self.ws.ensure(need + (TILE_RESERVE if self.cfg.engine == "tiles" else 0))
-            d_nets = self._plan_nets[:max(m, 1)]
-            d_off = self._plan_off[:max(m, 1)]
+        d_nets, d_off = self._plan_nets, self._plan_off
+        reserve = TILE_RESERVE if self.cfg.engine == "tiles" else 0
+
+        def stage(part: list[np.ndarray], base: int):
+            """Upload a part of the plan at offset ``base`` of the plan
+            buffers; its launch calls."""
+            flat_nets, flat_off, wave_ptr, wave_max, wave_ws, need = _flatten_waves(part, self.nets.roi_nodes)
+            m = len(flat_nets)
+            if base + m > len(self._plan_nets_h):
+                raise ValueError("a launch plan lists more nets than the job has")
+            self._plan_nets_h.numpy()[base:base + m] = flat_nets
+            self._plan_off_h.numpy()[base:base + m] = flat_off
+            self.ws.ensure(need + reserve)
             if m:
-                d_nets[:m].copy_(self._plan_nets_h[:m], non_blocking=True)
-                d_off[:m].copy_(self._plan_off_h[:m], non_blocking=True)
+                d_nets[base:base + m].copy_(self._plan_nets_h[base:base + m], non_blocking=True)
+                d_off[base:base + m].copy_(self._plan_off_h[base:base + m], non_blocking=True)
+            return self._route_calls(params, rcfg, flat_nets, wave_ptr, wave_max, wave_ws, d_nets[base:],
+                                     d_off[base:]), m
+
+        n_waves = len(waves)
+        with torch.cuda.stream(stream):
             self._route_stats.zero_()
             ev[0].record(stream)
             self.lat.reprice(params, stream)
             rcfg = self.route_cfg()
             ev[1].record(stream)
             exec_graph = None
-            calls = self._route_calls(params, rcfg, flat_nets, wave_ptr, wave_max, wave_ws, d_nets, d_off)
+            calls, m = stage(waves, 0)
             if graph and len(waves):
                 cs = C.c_void_p()
                 _lib.call("orx_graph_begin", C.byref(cs))
@@ -707,6 +726,13 @@ class DevicePathFinder:
             else:
                 for entry, route_args in calls:
                     _lib.call(entry, *route_args, s)
+                if rest is not None:
+                    tail = self.plan(rest, mode)
+                    calls, m2 = stage(tail, m)
+                    for entry, route_args in calls:
+                        _lib.call(entry, *route_args, s)
+                    n_waves += len(tail)
+                    m += m2
             ev[2].record(stream)
             _lib.call("orx_lattice_end_iteration", C.byref(self.lat.struct), C.byref(params),
                       self._iter_out.data_ptr(), s)
@@ -717,8 +743,7 @@ class DevicePathFinder:
                 self._stage_net_overuse()     # input of the next iteration's order
             done = torch.cuda.Event()
             done.record(stream)
-        self._pending = _Pending(it, pres, len(waves), t0, ev, done, exec_graph, (d_nets, d_off),
-                                 int(sum(len(w) for w in waves)))
+        self._pending = _Pending(it, pres, n_waves, t0, ev, done, exec_graph, (d_nets, d_off), m)
         return self._pending
 
     def ready(self) -> bool:
