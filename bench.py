@@ -67,7 +67,8 @@ TRAFFIC_FILES = {"tiles": REPO / "profiles" / "route_traffic_uneven_800x800x20.j
 ROUTE_KERNELS = {"tiles": "the route kernels of one iteration: orx_route_tiles (tile_fill, tile_prep, tile_step, "
                           "tile_drain, tile_post) for waves whose largest ROI exceeds 16384 nodes, orx_route_waves "
                           "(ws_fill, route_wave_kernel) for the others",
-                 "wave": "route_wave_kernel"}
+                 "wave": "the route kernels of one iteration: orx_route_waves (ws_fill, route_wave_kernel) for "
+                         "every wave"}
 
 
 def _log(msg: str) -> None:
@@ -734,9 +735,8 @@ def main() -> int:
         cpu_all = cpu_baseline_all_cores(nl, cfg)
 
     traffic, traffic_src = _traffic(args.engine)
-    # one launch: one iteration's route kernels for the tile engine, one
-    # route_wave_kernel dispatch (a wave) for the wave engine
-    n_dispatch = args.steps if args.engine == "tiles" else waves
+    # one launch: one iteration's route kernels (either engine)
+    n_dispatch = args.steps
     out = {
         "metric": "nets routed/s (batched disjoint-ROI PathFinder iteration)",
         "value": routed / wall,
