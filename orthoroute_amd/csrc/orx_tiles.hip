@@ -286,8 +286,10 @@ __device__ __forceinline__ uint64_t scan_step(uint64_t& k, uint64_t& c, float w,
 // b0 + i * ST (i = 0 .. N-1 forward, N-1 .. 0 backward), weights at float
 // WO of each node's float4, in-slot SLOT; the carry c enters the first node.
 // Chunks of 4 nodes: a chunk's LDS reads go out together (immediate
-// offsets), its relaxations run on registers, its changed keys are written
-// back; the scheduling barrier keeps the next chunk's reads from piling up
+// offsets), its relaxations run on registers, its keys are written back
+// (an unchanged key rewrites itself: cheaper than a branch per node; a node
+// belongs to one line or column, so no other lane touches it in the phase);
+// the scheduling barrier keeps the next chunk's reads from piling up
 // registers.  MT: targets (tg bits) have no out-arcs.  Returns the changed
 // mask (bit i = node i).
 template <int N, int ST, int WO, uint32_t SLOT, bool FWD, bool MT>
@@ -314,7 +316,7 @@ __device__ __forceinline__ uint32_t scan_fixed(uint64_t* K, const float* wf, int
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = FWD ? c0 + q : N - 1 - c0 - q;
-            if (c0 + q < N && ((cm >> i) & 1u)) K[b0 + i * ST] = k[q];
+            if (c0 + q < N) K[b0 + i * ST] = k[q];
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -350,7 +352,7 @@ __device__ __forceinline__ uint32_t scan_var(uint64_t* K, const float* wf, int32
         for (int q = 0; q < 4; ++q) {
             const int32_t sq = c0 + q;
             const int32_t i = FWD ? sq : m - 1 - sq;
-            if (sq < m && ((cm >> i) & 1u)) K[b0 + i * ST] = k[q];
+            if (sq < m) K[b0 + i * ST] = k[q];
         }
     }
     return cm;
