@@ -102,11 +102,13 @@ __host__ __device__ inline int32_t tile_ty(int32_t L) { return class_ty(layer_cl
 struct TileLds {
     int32_t ty, cells, keys, w4, cm, cp, tg, mk, bytes;
 };
-// Mark words: a row ly of the H layers is bit z of mk[kMkHRow + ly], a
-// column lx of the V layers bit z of mk[kMkVCol + lx]; via column (lx, ly)
-// is bit lx of mk[kMkViaRow + ly] (marked by an H-row scan) or bit ly of
-// mk[kMkViaCol + lx] (by a V-column scan).
-constexpr int32_t kMkHRow = 0, kMkVCol = 16, kMkViaRow = 32, kMkViaCol = 48, kMkWords = 64;
+// Mark words, in two copies each (buffer b at + b * kMkSet): a row ly of the
+// H layers is bit z of mk[kMkHRow + ly], a column lx of the V layers bit z
+// of mk[kMkVCol + lx]; via column (lx, ly) is bit lx of mk[kMkViaRow + ly]
+// (marked by an H-row scan) or bit ly of mk[kMkViaCol + lx] (by a V-column
+// scan).  Phase j reads one copy and sets bits in a copy of the other kind;
+// the copy it reads is zeroed in phase j + 1, when no phase reads or sets it.
+constexpr int32_t kMkHRow = 0, kMkVCol = 16, kMkViaRow = 64, kMkViaCol = 80, kMkSet = 32, kMkWords = 128;
 __host__ __device__ inline TileLds tile_lds(int32_t L) {
     TileLds t;
     t.ty = tile_ty(L);
@@ -398,9 +400,12 @@ constexpr int32_t kPerLane = (16 * 16 * 20 + kTileBlock - 1) / kTileBlock;
 // holds a seed on the tile's first visit) or, in the first phase, when its
 // halo carry improves its end node; the via phase likewise scans the marked
 // columns.  A scan that changes nodes marks, with one LDS atomic, the lines
-// of the other phase through them, and the neighbour whose halo it is.  No
-// phase sets the marks it reads, so a scanner clears its own mark bit
-// without a race.  ZL: the H layers in order, then (from 32) the V layers.
+// of the other phase through them, and the neighbour whose halo it is.  The
+// marks are double-buffered (h = (j / 2) & 1): a planar phase reads planar
+// copy h and sets via copy h, a via phase reads via copy h and sets planar
+// copy h ^ 1; each phase zeroes the copy the phase before it read, so no
+// mark is cleared bit by bit.  ZL: the H layers in order, then (from 32) the
+// V layers.
 // Returns whether a key changed.
 template <int NT, int LC, bool MT>
 __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t L, int32_t tw,
@@ -412,7 +417,12 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
     const uint32_t tid = threadIdx.x;
     const uint32_t hm = S.hmask & (L < 32 ? (1u << L) - 1u : ~0u);
     uint32_t chg = 0;
+    const int32_t h = (j >> 1) & 1;
     if ((j & 1) == 0) {
+        // the via copy the via phase before read: free until planar phase j + 2
+        if (j >= 2 && (int32_t)tid < kMkSet) MK[kMkViaRow + (h ^ 1) * kMkSet + tid] = 0u;
+        uint32_t* const PR = MK + h * kMkSet;         // planar marks read
+        uint32_t* const VW = MK + h * kMkSet;         // via marks set (offset by kMkViaRow below)
         const int32_t nh = __popc(hm);
         const int32_t hl = (nh * kLines + 63) / 64 * 64, pl = hl + (L - nh) * kLines;
 #pragma unroll 1
@@ -425,13 +435,11 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
             const int32_t li = z * kLines + l;
             const int32_t b0 = horiz ? (z * TY + l) * kPitch : z * ZS + l;
             const int32_t st = horiz ? 1 : kPitch;
-            uint32_t* mw = MK + (horiz ? kMkHRow : kMkVCol) + l;
             const uint32_t bit = 1u << z;
             const uint64_t cmi = CM[li], cpi = CP[li];
-            const bool marked = (*mw & bit) != 0;
+            const bool marked = (PR[(horiz ? kMkHRow : kMkVCol) + l] & bit) != 0;
             const bool need = marked || (j == 0 && (cmi < K[b0] || cpi < K[b0 + (n - 1) * st]));
             if (!need) continue;
-            if (marked) atomicAnd(mw, ~bit);
             uint32_t tg = 0;
             if (MT)
                 for (int32_t i = 0; i < n; ++i) tg |= (uint32_t)TG[b0 + i * st] << i;
@@ -441,15 +449,18 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
                 if (cm & 1u) sh.act[horiz ? 0 : 2] = 1u;
                 if ((cm >> (n - 1)) & 1u) sh.act[horiz ? 1 : 3] = 1u;
                 // H row l: via columns (lx in cm, l); V column l: (l, ly in cm)
-                atomicOr(MK + (horiz ? kMkViaRow : kMkViaCol) + l, cm);
+                atomicOr(VW + (horiz ? kMkViaRow : kMkViaCol) + l, cm);
                 chg |= cm;
             }
         }
-    } else if ((int32_t)tid < kTX * TY) {
+    } else {
+        // the planar copy the planar phase before read: free until via phase j + 2
+        if ((int32_t)tid < kMkSet) MK[h * kMkSet + tid] = 0u;
+        const uint32_t* const VR = MK + h * kMkSet;       // via marks read (offset by kMkViaRow below)
+        uint32_t* const PW = MK + (h ^ 1) * kMkSet;       // planar marks set
         const int32_t lx = (int32_t)tid & (kTX - 1), ly = (int32_t)tid >> 4;
-        if (lx < tw && ly < th && (((MK[kMkViaRow + ly] >> lx) | (MK[kMkViaCol + lx] >> ly)) & 1u)) {
-            atomicAnd(MK + kMkViaRow + ly, ~(1u << lx));
-            atomicAnd(MK + kMkViaCol + lx, ~(1u << ly));
+        if ((int32_t)tid < kTX * TY && lx < tw && ly < th &&
+            (((VR[kMkViaRow + ly] >> lx) | (VR[kMkViaCol + lx] >> ly)) & 1u)) {
             const int32_t b0 = ly * kPitch + lx;
             uint32_t tg = 0;
             if (MT)
@@ -461,8 +472,8 @@ __device__ __forceinline__ bool tile_phase(const TileStep& S, int32_t j, int32_t
                 if (ly == 0 && (cm & ~hm)) sh.act[2] = 1u;
                 if (ly == th - 1 && (cm & ~hm)) sh.act[3] = 1u;
                 // changed nodes of H layers lie on row ly, of V layers on column lx
-                if (cm & hm) atomicOr(MK + kMkHRow + ly, cm & hm);
-                if (cm & ~hm) atomicOr(MK + kMkVCol + lx, cm & ~hm);
+                if (cm & hm) atomicOr(PW + kMkHRow + ly, cm & hm);
+                if (cm & ~hm) atomicOr(PW + kMkVCol + lx, cm & ~hm);
                 chg = cm;
             }
         }
