@@ -8,7 +8,9 @@
 //   2. relaxes the tile to a local fixpoint with line scans: a planar phase
 //      scans every planar line of the tile forward and backward (one lane per
 //      line, the line in registers), a via phase scans every via column up
-//      and down; the phases alternate until one changes nothing;
+//      and down; the phases alternate until one changes nothing (on routing
+//      waves at most kRoutePhaseCap phases: a visit still changing keys then
+//      stops and its tile is visited again, from scratch, in the next step);
 //   3. writes the keys that improved back and activates the neighbour tiles
 //      whose halo it changed, for the next step.
 // Steps repeat until no tile is active.  Every relaxation is one arc of the
