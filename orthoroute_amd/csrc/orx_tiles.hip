@@ -279,10 +279,23 @@ struct TileShared {
 
 // One relaxation step of a scan: node key k receives the carry c (kept if
 // smaller), then offers its own relaxation over weight w to the next node.
-__device__ __forceinline__ uint64_t scan_step(uint64_t& k, uint64_t& c, float w, uint32_t slot) {
+//
+// The carry's own infinity is tracked in cinf, so whether the kept key is
+// +inf ("no path yet": both the carry and the node's key are +inf) comes
+// from flags known before the compare instead of a test of the kept key's
+// bits after it; the offered key is relax_key's, bit for bit.  The hop
+// count steps as (lo | 0xff) + 1: the low byte (the in-slot) carries into
+// the hop field, as (lo & ~0xff) + 0x100 does.
+__device__ __forceinline__ uint64_t scan_step(uint64_t& k, uint64_t& c, bool& cinf, float w, uint32_t slot) {
+    const bool kinf = k == ORX_KEY_INF;
     const bool lt = c < k;
     k = lt ? c : k;
-    c = relax_key(k, w, slot);
+    const bool bad = (cinf && kinf) || isinf(w);
+    const uint32_t hi = (uint32_t)(k >> 32), lo = (uint32_t)k;
+    const uint32_t nhi = bad ? 0xFFFFFFFFu : __float_as_uint(__fadd_rn(__uint_as_float(hi), w));
+    const uint32_t nlo = bad ? 0xFFFFFFFFu : (lo | 0xFFu) + (1u + slot);
+    c = ((uint64_t)nhi << 32) | nlo;
+    cinf = bad;
     return lt;
 }
 
@@ -299,6 +312,7 @@ __device__ __forceinline__ uint64_t scan_step(uint64_t& k, uint64_t& c, float w,
 template <int N, int ST, int WO, uint32_t SLOT, bool FWD, bool MT>
 __device__ __forceinline__ uint32_t scan_fixed(uint64_t* K, const float* wf, int32_t b0, uint32_t tg, uint64_t c) {
     uint32_t cm = 0;
+    bool cinf = c == ORX_KEY_INF;
 #pragma unroll
     for (int c0 = 0; c0 < N; c0 += 4) {
         uint64_t k[4];
@@ -315,7 +329,7 @@ __device__ __forceinline__ uint32_t scan_fixed(uint64_t* K, const float* wf, int
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = FWD ? c0 + q : N - 1 - c0 - q;
-            if (c0 + q < N && scan_step(k[q], c, w[q], SLOT)) cm |= 1u << i;
+            if (c0 + q < N && scan_step(k[q], c, cinf, w[q], SLOT)) cm |= 1u << i;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -333,6 +347,7 @@ template <int ST, int WO, uint32_t SLOT, bool FWD, bool MT>
 __device__ __forceinline__ uint32_t scan_var(uint64_t* K, const float* wf, int32_t b0, int32_t m, uint32_t tg,
                                              uint64_t c) {
     uint32_t cm = 0;
+    bool cinf = c == ORX_KEY_INF;
 #pragma unroll 1
     for (int32_t c0 = 0; c0 < m; c0 += 4) {
         uint64_t k[4];
@@ -350,7 +365,7 @@ __device__ __forceinline__ uint32_t scan_var(uint64_t* K, const float* wf, int32
         for (int q = 0; q < 4; ++q) {
             const int32_t sq = c0 + q;
             const int32_t i = FWD ? sq : m - 1 - sq;
-            if (sq < m && scan_step(k[q], c, w[q], SLOT)) cm |= 1u << i;
+            if (sq < m && scan_step(k[q], c, cinf, w[q], SLOT)) cm |= 1u << i;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
