@@ -306,35 +306,47 @@ __device__ __forceinline__ uint64_t scan_step(uint64_t& k, uint64_t& c, bool& ci
 // offsets), its relaxations run on registers, its keys are written back
 // (an unchanged key rewrites itself: cheaper than a branch per node; a node
 // belongs to one line or column, so no other lane touches it in the phase);
-// the scheduling barrier keeps the next chunk's reads from piling up
-// registers.  MT: targets (tg bits) have no out-arcs.  Returns the changed
-// mask (bit i = node i).
+// the next chunk's reads are issued before the current chunk is relaxed,
+// and scheduling barriers keep the chunks from piling up registers.  MT:
+// targets (tg bits) have no out-arcs.  Returns the changed mask (bit i =
+// node i).
 template <int N, int ST, int WO, uint32_t SLOT, bool FWD, bool MT>
 __device__ __forceinline__ uint32_t scan_fixed(uint64_t* K, const float* wf, int32_t b0, uint32_t tg, uint64_t c) {
     uint32_t cm = 0;
     bool cinf = c == ORX_KEY_INF;
-#pragma unroll
-    for (int c0 = 0; c0 < N; c0 += 4) {
-        uint64_t k[4];
-        float w[4];
+    constexpr int NC = (N + 3) / 4;
+    uint64_t k[2][4];
+    float w[2][4];
+    // chunk c's reads are issued before chunk c - 1 is relaxed (two register
+    // sets), so their LDS latency overlaps that chunk's dependent chain
+    auto fetch = [&](int c0, uint64_t* kk, float* ww) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = FWD ? c0 + q : N - 1 - c0 - q;
             if (c0 + q < N) {
-                k[q] = K[b0 + i * ST];
-                w[q] = wf[4 * i * ST + WO];
-                if (MT && ((tg >> i) & 1u)) w[q] = INFINITY;
+                kk[q] = K[b0 + i * ST];
+                ww[q] = wf[4 * i * ST + WO];
+                if (MT && ((tg >> i) & 1u)) ww[q] = INFINITY;
             }
         }
+    };
+    fetch(0, k[0], w[0]);
+#pragma unroll
+    for (int ci = 0; ci < NC; ++ci) {
+        const int c0 = 4 * ci;
+        uint64_t* kk = k[ci & 1];
+        float* ww = w[ci & 1];
+        if (ci + 1 < NC) fetch(c0 + 4, k[(ci + 1) & 1], w[(ci + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = FWD ? c0 + q : N - 1 - c0 - q;
-            if (c0 + q < N && scan_step(k[q], c, cinf, w[q], SLOT)) cm |= 1u << i;
+            if (c0 + q < N && scan_step(kk[q], c, cinf, ww[q], SLOT)) cm |= 1u << i;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = FWD ? c0 + q : N - 1 - c0 - q;
-            if (c0 + q < N) K[b0 + i * ST] = k[q];
+            if (c0 + q < N) K[b0 + i * ST] = kk[q];
         }
         __builtin_amdgcn_sched_barrier(0);
     }
