@@ -577,8 +577,8 @@ def main() -> int:
     ap.add_argument("--lds-nodes", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs the path on a "
-              f"single GPU (multi-GPU sharding is a design, not an implementation)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the route path runs on a "
+              f"single GPU (sharding it across GPUs is designed, not implemented)", file=sys.stderr)
         return 2
     if args.copy_probe:
         copy_probe()
