@@ -184,13 +184,10 @@ __device__ __forceinline__ void reprice_in_arcs(const Lat& g, const Params& P, i
         int64_t n = lat_nbr(g, v, t);
         if (n < 0) continue;
         int64_t a = 4 * n + (3 - t);
-        float b = g.base[a];
-        float w = INFINITY;
-        if (!isinf(b)) {
-            int32_t o = __hip_atomic_load(g.occ + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            w = __fadd_rn(arc_cost(b, g.hist[a], o, P), q);
-        }
-        g.wcost[a] = w;
+        // the arc's three loads go out together (no branch between them)
+        const float b = g.base[a], hst = g.hist[a];
+        const int32_t o = __hip_atomic_load(g.occ + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        g.wcost[a] = isinf(b) ? INFINITY : __fadd_rn(arc_cost(b, hst, o, P), q);
     }
 }
 

@@ -87,6 +87,7 @@ class CpuBuildPathFinder:
         order = net_order(it, self.distance, over, hops)
         wave, sched, dep_ptr, dep_ids = plan_iteration(self.nl.box, order, self.spec.X, self.spec.Y, self.R)
         check_schedule(sched, dep_ptr, dep_ids)
+        self.last_wave = wave
         if self.mode == 1:
             sched = order.astype(np.int32)
         p = np.float32(self.schedule[it - 1])

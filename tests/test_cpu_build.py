@@ -347,8 +347,10 @@ def test_tile_queue_deadline_commits_nothing(driver, tmp_path, monkeypatch):
     """The queue form's bounded wait: with a deadline far below a visit's
     time (ORX_TILE_RUN_SECONDS), waiting workgroups give up and flag the
     launch; every net of such a wave then fails with ORX_E_OVERFLOW and
-    commits nothing, the launch still ends, and nets of waves that finished
-    equal the oracle's."""
+    commits nothing, the launch still ends, and the nets of the waves before
+    the first failed one equal the oracle's (later waves see the costs
+    without the failed nets' paths, so they may route differently; which
+    waves miss the deadline depends on the host's load)."""
     from tests.cpu_build.harness import CpuBuildPathFinder
     monkeypatch.setenv("HIPCPU_CONCURRENT_WG", "4")
     monkeypatch.setenv("ORX_TILE_RUN_SECONDS", "0.000001")
@@ -363,8 +365,9 @@ def test_tile_queue_deadline_commits_nothing(driver, tmp_path, monkeypatch):
     assert (st == -3).sum() >= 1
     failed = np.flatnonzero(st == -3)
     assert (c.path_len[failed] == 0).all()
+    first_failed_wave = int(c.last_wave[failed].min())
     for k, (a, b) in enumerate(zip(c.paths(), o.paths())):
-        if st[k] == 0:
+        if st[k] == 0 and c.last_wave[k] < first_failed_wave:
             assert np.array_equal(a, b), k
     # occupancy is exactly what the committed paths hold
     occ = np.zeros_like(c.occ)
