@@ -81,6 +81,15 @@ def test_uneven_small_bit_exact(device, lds_nodes, delta, astar):
     _run_both(nl, cfg, 4, "batched", device)
 
 
+@pytest.mark.parametrize("head_nets", [0, 1, 37, 128])
+def test_split_planning_bit_exact(device, head_nets):
+    """The head of the order planned and launched before the rest (its waves
+    after the head's) routes like the sequential order, for any head size."""
+    nl = uneven(96, 80, 8, 300, seed=13)
+    cfg = RouterConfig(iterations=3, head_nets=head_nets)
+    _run_both(nl, cfg, 3, "batched", device)
+
+
 @pytest.mark.parametrize("lds_nodes", [0, 16384])
 def test_wide_bucket_large_block_bit_exact(device, lds_nodes):
     """The tuned launch shape (wide bucket, 1024 lanes) on a congested board."""
