@@ -277,6 +277,20 @@ struct TileShared {
     unsigned char zl[64];   // the H layers in order, then (from 32) the V layers
 };
 
+// The layer lists of TileShared, filled once per launch (they depend only on
+// the layer count and the H mask): lane z < L writes layer z at its rank
+// among the layers of its kind.  The first barrier of a visit orders the
+// writes before any read.
+__device__ __forceinline__ void tile_layer_lists(const TileStep& S, int32_t L, TileShared& sh) {
+    const int32_t z = (int32_t)threadIdx.x;
+    if (z < L) {
+        const uint32_t h = (S.hmask >> z) & 1u;
+        const uint32_t below = (1u << z) - 1u;
+        const int32_t rank = __popc((h ? S.hmask : ~S.hmask) & below);
+        sh.zl[(h ? 0 : 32) + rank] = (unsigned char)z;
+    }
+}
+
 // One relaxation step of a scan: node key k receives the carry c (kept if
 // smaller), then offers its own relaxation over weight w to the next node.
 //
@@ -560,11 +574,6 @@ __device__ void tile_visit(const TileStep& S, uint32_t s, uint32_t t, uint32_t p
         }
         sh.changed[0] = sh.changed[1] = sh.changed[2] = 0;
         sh.act[0] = sh.act[1] = sh.act[2] = sh.act[3] = 0;
-        int32_t nh = 0, nv = 0;
-        for (int32_t z = 0; z < L; ++z) {
-            if ((S.hmask >> z) & 1u) sh.zl[nh++] = (unsigned char)z;
-            else sh.zl[32 + nv++] = (unsigned char)z;
-        }
     }
     // 1. every global load of the visit at once: the weights straight into
     // LDS (16 bytes per lane, each wave filling 64 consecutive node indices;
@@ -928,6 +937,7 @@ __global__ __launch_bounds__(kTileBlock) void tile_step_kernel(TileStep S, uint3
         return;
     }
     const uint32_t pin = k & 1u, pout = pin ^ 1u;
+    tile_layer_lists(S, LC ? LC : S.L, sh);
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
         tile_visit<kTileBlock, LC, false>(S, S.item_s[pin][i], S.item_t[pin][i], pin, pout, S.cnt + (k + 1) % 3, 0u,
                                           reinterpret_cast<unsigned char*>(lds_mem), sh);
@@ -941,6 +951,7 @@ __global__ __launch_bounds__(kTileBlock) void tile_drain_kernel(TileStep S, uint
     ORX_DYNAMIC_LDS(uint64_t, lds_mem);
     ORX_SHARED(TileShared, sh);
     uint32_t visits = 0;
+    tile_layer_lists(S, LC ? LC : S.L, sh);
     for (uint32_t k = k0;; ++k) {
         const uint32_t n = gload_u32(S.cnt + k % 3);
         __syncthreads();   // every lane has read n before lane 0 clears a counter
@@ -972,6 +983,7 @@ __global__ __launch_bounds__(kTileBlock) void tile_run_kernel(TileStep S, uint64
     ORX_SHARED(uint64_t, s_item);
     const uint32_t epoch = gload_u32(S.cnt + 3);
     const uint64_t deadline = wall_clock64() + deadline_ticks;
+    tile_layer_lists(S, LC ? LC : S.L, sh);
     for (;;) {
         if (threadIdx.x == 0) s_item = ring_pop(S, epoch, deadline);
         __syncthreads();
