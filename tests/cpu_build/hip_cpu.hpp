@@ -21,6 +21,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <barrier>
 #include <cmath>
 #include <cstdlib>
@@ -134,6 +135,56 @@ inline unsigned mbcnt_hi(unsigned mask, unsigned acc) {
     return acc + (unsigned)__builtin_popcount(m);
 }
 
+// Worker threads kept across launches (a launch of a 512-lane workgroup
+// would otherwise start and join 512 OS threads).  run(n, f) calls f(i) on
+// threads 0..n-1 and returns when every call has returned; launches come
+// from one host thread, one at a time.  The pool is never destroyed: its
+// idle threads end with the process.
+class Pool {
+public:
+    void run(unsigned n, const std::function<void(unsigned)>& f) {
+        std::unique_lock<std::mutex> lk(m_);
+        while (th_.size() < n) {
+            const unsigned i = (unsigned)th_.size();
+            th_.emplace_back([this, i]() { work(i); });
+            th_.back().detach();
+        }
+        job_ = &f;
+        active_ = n;
+        left_ = n;
+        ++gen_;
+        go_.notify_all();
+        done_.wait(lk, [this]() { return left_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void work(unsigned i) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            go_.wait(lk, [&]() { return gen_ != seen; });
+            seen = gen_;
+            if (i >= active_) continue;
+            const std::function<void(unsigned)>* f = job_;
+            lk.unlock();
+            (*f)(i);
+            lk.lock();
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    std::vector<std::thread> th_;
+    const std::function<void(unsigned)>* job_ = nullptr;
+    uint64_t gen_ = 0;
+    unsigned active_ = 0, left_ = 0;
+};
+inline Pool& pool() {
+    static Pool* p = new Pool;
+    return *p;
+}
+
 template <typename K, typename... A>
 void launch(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
     g_bdim = {block.x, 1, 1};
@@ -148,7 +199,7 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
     // has ended, as with a fresh set of threads per step.
     const unsigned nb = std::min(conc, grid.x);
     std::vector<std::unique_ptr<Block>> blks(nb);
-    std::barrier<> pool((ptrdiff_t)nb * block.x);
+    std::barrier<> step((ptrdiff_t)nb * block.x);
     auto build = [&](unsigned b0) {
         for (unsigned j = 0; j < nb; ++j) {
             blks[j].reset();
@@ -167,23 +218,21 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds, A... args) {
             blks[j] = std::move(blk);
         }
     };
-    std::vector<std::thread> th;
-    th.reserve((size_t)nb * block.x);
-    for (unsigned j = 0; j < nb; ++j)
-        for (unsigned t = 0; t < block.x; ++t)
-            th.emplace_back([&, j, t]() {
-                for (unsigned b0 = 0; b0 < grid.x; b0 += nb) {
-                    if (j == 0 && t == 0) build(b0);
-                    pool.arrive_and_wait();
-                    if (blks[j]) {
-                        t_thread = {t, 0, 0};
-                        t_blk = blks[j].get();
-                        kernel(args...);
-                    }
-                    pool.arrive_and_wait();
-                }
-            });
-    for (auto& x : th) x.join();
+    const std::function<void(unsigned)> body = [&](unsigned i) {
+        const unsigned j = i / block.x, t = i % block.x;
+        for (unsigned b0 = 0; b0 < grid.x; b0 += nb) {
+            if (j == 0 && t == 0) build(b0);
+            step.arrive_and_wait();
+            if (blks[j]) {
+                t_thread = {t, 0, 0};
+                t_blk = blks[j].get();
+                kernel(args...);
+            }
+            step.arrive_and_wait();
+        }
+        t_blk = nullptr;
+    };
+    pool().run(nb * block.x, body);
 }
 }  // namespace hipcpu
 
