@@ -13,7 +13,7 @@ columns, overuse).
 The kernels themselves are checked against the CPU oracle in
 tests/test_gpu_parity.py (including the largest ROIs of the 800x800x20
 board and a backplane subset) and at full size in tests/test_gpu_configs.py
-(800x800x20 iterations 1-2, the 1000x1000x12 query); these tests pin
+(800x800x20 iterations 1-5, the 1000x1000x12 query); these tests pin
 batching against sequential order on the device over more iterations.
 """
 import numpy as np
